@@ -1,0 +1,130 @@
+/*
+ * pic_hip.h -- C ABI of the MI355X (gfx950) time-stepping backend for the
+ * pulsed ion chamber: Gaussian track deposition, the Lax-Wendroff drift +
+ * diffusion + recombination sweep of both carrier species, and the scored
+ * charge reduction.
+ *
+ * Plain C types only: opaque handles, plain pointers, sizes. Every entry point
+ * returns 0 on success or a negative PIC_E_* code; pic_last_error() returns
+ * the message of the last failure on the calling thread. Host pointers are
+ * copied during the call and never retained. One handle owns one HIP stream
+ * and its device arrays; a handle must not be used from two threads at once.
+ *
+ * Grid layout (same as the reference): each species is a C-contiguous
+ * (no_xy, no_xy, nzb) array, k (the field axis) fastest. Fields are stored in
+ * FP64 (PIC_FP64) or FP32 (PIC_FP32); every scored sum, the recombination
+ * product and the deposited density are FP64 in both modes.
+ *
+ * The reference binds its backends as Python functions; the entry points
+ * below replace the pieces of those functions named in each comment
+ * (paths relative to the reference checkout).
+ */
+#ifndef PIC_HIP_H
+#define PIC_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PIC_OK 0
+#define PIC_E_INVALID -1  /* bad argument or state */
+#define PIC_E_NOMEM -2    /* device allocation or size check failed */
+#define PIC_E_HIP -3      /* HIP runtime error */
+#define PIC_E_NODEVICE -4 /* no usable GPU */
+
+#define PIC_FP64 0
+#define PIC_FP32 1
+
+#define PIC_WALL_ABSORBING 0
+#define PIC_WALL_REFLECTING 1
+
+typedef struct pic_sim pic_sim;
+
+/* Everything the loop reads from SimulationConfig (config.py:180-401,
+ * scheme_coefficients config.py:466-482, alpha_dt solver_numba.py:273-274). */
+typedef struct pic_params {
+    int64_t no_xy;            /* lateral voxels per side */
+    int64_t nzb;              /* no_z_with_buffer: voxels along the field axis */
+    int64_t nze;              /* no_z_electrode: first gap layer */
+    int64_t no_z;             /* gap layers */
+    double mid_xy;            /* grid centre, no_xy/2.0 */
+    double scoring_radius_sq; /* (i-mid)^2 + (j-mid)^2 < this is scored */
+    double p_lat, p_zm, p_zp, p_cen; /* positive-ion stencil weights */
+    double n_lat, n_zm, n_zp, n_cen; /* negative-ion stencil weights */
+    double alpha_dt;          /* recombination coefficient times dt */
+    double gaussian_factor;   /* N0 / (pi b^2) */
+    double h2_over_b2;        /* voxel area over squared track radius */
+    double cutoff_voxels;     /* deposition box half width */
+    int32_t wall;             /* PIC_WALL_* */
+    int32_t reserved;         /* must be 0 */
+} pic_params;
+
+/* Message of the last failed call on this thread (always NUL-terminated). */
+int pic_last_error(char *buf, size_t len);
+
+/* Device name and free/total memory of the current device
+ * (replaces the device checks of solver_cuda.py:141-172, 589-642). */
+int pic_device_info(char *name, size_t name_len, size_t *free_bytes, size_t *total_bytes);
+
+/* Device bytes a handle needs for these params and precision, before any
+ * tracks are uploaded (carrier arrays solver_cuda.py:420-423 plus scratch). */
+int pic_mem_required(const pic_params *p, int precision, size_t *bytes);
+
+/* Allocate a zeroed state on the current device (solver_cuda.py:742-772). */
+int pic_create(const pic_params *p, int precision, pic_sim **out);
+int pic_destroy(pic_sim *sim);
+
+/* Upload every track of a run once: xs/ys of n_tracks centres in arrival
+ * order, and n_steps+1 offsets so that step s deposits
+ * xs[offsets[s]:offsets[s+1]] (solver_cuda.py:791-816 does this per step).
+ * Replaces any earlier upload; per-step results are reset to zero. */
+int pic_upload_tracks(pic_sim *sim, const double *xs, const double *ys, int64_t n_tracks,
+                      const int64_t *offsets, int64_t n_steps);
+
+/* Run time steps [first, first+count) of the uploaded schedule: per step,
+ * gather-deposit that step's tracks, sweep, wall update, reduce, swap
+ * (the loop body of solver_cuda.py:791-865). Asynchronous on the handle's
+ * stream; results land in a device (n_steps, 4) array. */
+int pic_run_steps(pic_sim *sim, int64_t first, int64_t count);
+
+/* Copy per-step (injected, recombined, total+, total-) rows [first,
+ * first+count) to host (n x 4 doubles). Synchronises the stream. */
+int pic_read_per_step(pic_sim *sim, int64_t first, int64_t count, double *out);
+
+/* Kernel-level calls (the granularity tests/test_charge_conservation.py
+ * drives _insert_track_numba / _lax_wendroff_step_numba at):
+ * deposit m tracks into the current arrays and return the scored injected
+ * charge (solver_numba_parallel.py:140-277 semantics, ascending track order
+ * per voxel) ... */
+int pic_deposit(pic_sim *sim, const double *xs, const double *ys, int64_t m, double *inserted);
+/* ... and sweep once with the wall update and buffer swap, returning
+ * (recombined, total+, total-) (solver_numba_parallel.py:280-418, 686-695). */
+int pic_step(pic_sim *sim, double out3[3]);
+
+/* Current carrier arrays to host as FP64, no_xy*no_xy*nzb each (either may be
+ * NULL). Synchronises. (solver_cuda.py:882-887) */
+int pic_read_fields(pic_sim *sim, double *pos, double *neg);
+/* Overwrite the current (and next) arrays from host FP64 data (test hook). */
+int pic_write_fields(pic_sim *sim, const double *pos, const double *neg);
+
+/* Block until all work queued on the handle's stream is done. */
+int pic_synchronize(pic_sim *sim);
+
+/* Sweep launch shape, for tuning: rows per tile and planes per block chunk
+ * (0 = automatic). Takes effect at the next launch. */
+int pic_set_tiling(pic_sim *sim, int rows_per_tile, int planes_per_chunk);
+
+/* Kernel timing of the last pic_run_steps call when timing is enabled:
+ * summed milliseconds of the sweep, deposition and wall/reduce kernels
+ * measured with HIP events on the handle's stream, and launch counts. */
+int pic_enable_timing(pic_sim *sim, int enable);
+int pic_get_timing(pic_sim *sim, double *sweep_ms, int64_t *sweep_launches, double *deposit_ms,
+                   int64_t *deposit_launches, double *other_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PIC_HIP_H */

@@ -1,0 +1,601 @@
+// C ABI of the gfx950 time-stepping backend (declared in include/pic_hip.h).
+//
+// One handle = one HIP stream + the four carrier arrays (current and next of
+// both species) + scratch. The time-step loop runs here, on the host side of
+// the library, enqueuing per step: gather deposition (in-pulse steps only),
+// the fused sweep, the ring update, and the per-step reduction into a device
+// (n_steps, 4) record. Nothing is copied back per step.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "pic_hip.h"
+#include "pic_kernels.hpp"
+
+namespace {
+
+thread_local char g_err[1024] = "";
+
+int fail(int code, const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                                  \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess) return fail(PIC_E_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+#define HIP_LAUNCH_CHECK()                                                                                  \
+    do {                                                                                                    \
+        hipError_t e_ = hipGetLastError();                                                                  \
+        if (e_ != hipSuccess) return fail(PIC_E_HIP, "kernel launch failed: %s", hipGetErrorString(e_)); \
+    } while (0)
+
+int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+struct SweepShape {
+    int tj = 0, chunk = 0, n_windows = 0, wpb = 0, n_kg = 0, n_jt = 0, n_ic = 0;
+    int64_t blocks() const { return (int64_t)n_kg * n_jt * n_ic; }
+};
+
+}  // namespace
+
+struct pic_sim {
+    pic_params p{};
+    int prec = PIC_FP64;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    size_t elem = 8;
+    int64_t n_vox = 0;
+    void *cur_p = nullptr, *cur_n = nullptr, *nxt_p = nullptr, *nxt_n = nullptr;
+    double *dens = nullptr;
+    double *sweep_part = nullptr;
+    int64_t sweep_part_cap = 0;
+    double *dep_part = nullptr;
+    int64_t dep_part_cap = 0;
+    double *scratch4 = nullptr;  // 8 doubles for the kernel-level calls
+    double *per_step = nullptr;
+    int64_t n_steps = 0;
+    double *xs = nullptr, *ys = nullptr;
+    int64_t n_tracks = 0;
+    std::vector<int64_t> offsets;
+    int force_tj = 0, force_chunk = 0;
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<int> ev_kind;  // 0 sweep, 1 deposit, 2 other
+    double t_sweep = 0, t_dep = 0, t_other = 0;
+    int64_t n_sweep = 0, n_dep = 0;
+};
+
+namespace {
+
+int sm_count(int device) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || v <= 0) v = 256;
+    return v;
+}
+
+// Rows per tile (registers) and planes per block: enough blocks to keep every
+// CU busy, as few halo rows and chunk prologues as that allows.
+SweepShape sweep_shape(const pic_sim *s) {
+    SweepShape sh;
+    const int64_t no_xy = s->p.no_xy, nzb = s->p.nzb;
+    sh.n_windows = (int)ceil_div(nzb - 2, pic::kOut);
+    sh.wpb = std::min(sh.n_windows, 8);
+    sh.n_kg = (int)ceil_div(sh.n_windows, sh.wpb);
+    const int64_t rows = no_xy - 2, planes = no_xy - 2;
+    const int cus = sm_count(s->device);
+    int tj = s->force_tj;
+    if (tj <= 0) {
+        // target ~16 waves per CU in flight; big grids get tall tiles
+        const int64_t waves_wanted = (int64_t)cus * 16;
+        // register budget: FP32 tiles up to 12 rows, FP64 up to 8, without spills
+        const int cand[] = {12, 8, 4, 2, 1};
+        tj = 1;
+        for (int c : cand) {
+            if (s->prec == PIC_FP64 && c > 8) continue;
+            const int64_t waves = ceil_div(rows, c) * sh.n_windows * std::max<int64_t>(1, planes / 16);
+            if (waves >= waves_wanted) {
+                tj = c;
+                break;
+            }
+        }
+    }
+    sh.tj = tj;
+    sh.n_jt = (int)ceil_div(rows, tj);
+    int chunk = s->force_chunk;
+    if (chunk <= 0) {
+        const int64_t waves_per_chunkset = (int64_t)sh.n_jt * sh.n_windows;
+        const int64_t waves_wanted = (int64_t)cus * 16;
+        int64_t n_ic = std::max<int64_t>(1, ceil_div(waves_wanted, waves_per_chunkset));
+        chunk = (int)std::max<int64_t>(8, ceil_div(planes, n_ic));
+    }
+    sh.chunk = (int)std::min<int64_t>(chunk, planes);
+    sh.n_ic = (int)ceil_div(planes, sh.chunk);
+    return sh;
+}
+
+template <typename S, typename A, int TJ>
+void launch_sweep_tj(const pic::SweepArgs &a, dim3 grid, dim3 block, hipStream_t st, bool dep) {
+    if (dep)
+        hipLaunchKernelGGL((pic::sweep_kernel<S, A, TJ, true>), grid, block, 0, st, a);
+    else
+        hipLaunchKernelGGL((pic::sweep_kernel<S, A, TJ, false>), grid, block, 0, st, a);
+}
+
+template <typename S, typename A>
+int launch_sweep_typed(const pic::SweepArgs &a, int tj, dim3 grid, dim3 block, hipStream_t st, bool dep) {
+    switch (tj) {
+        case 1: launch_sweep_tj<S, A, 1>(a, grid, block, st, dep); break;
+        case 2: launch_sweep_tj<S, A, 2>(a, grid, block, st, dep); break;
+        case 4: launch_sweep_tj<S, A, 4>(a, grid, block, st, dep); break;
+        case 8: launch_sweep_tj<S, A, 8>(a, grid, block, st, dep); break;
+        case 12:
+            if constexpr (sizeof(S) == 8) {
+                return fail(PIC_E_INVALID, "rows_per_tile 12 is FP32 only");
+            } else {
+                launch_sweep_tj<S, A, 12>(a, grid, block, st, dep);
+            }
+            break;
+        default: return fail(PIC_E_INVALID, "unsupported rows_per_tile %d (1, 2, 4, 8 or 12)", tj);
+    }
+    return PIC_OK;
+}
+
+void record(pic_sim *s, int kind) {
+    if (!s->timing) return;
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    hipEventRecord(e, s->stream);
+    s->ev_pool.push_back(e);
+    s->ev_kind.push_back(kind);
+}
+
+// sweep of the current arrays into the next ones (+ deposition if dep)
+int enqueue_sweep(pic_sim *s, bool dep) {
+    const SweepShape sh = sweep_shape(s);
+    const int64_t nblk = sh.blocks();
+    if (nblk > s->sweep_part_cap) return fail(PIC_E_INVALID, "sweep grid %lld exceeds partial buffer", (long long)nblk);
+    pic::SweepArgs a{};
+    a.pos = s->cur_p;
+    a.neg = s->cur_n;
+    a.pos_out = s->nxt_p;
+    a.neg_out = s->nxt_n;
+    a.dens = s->dens;
+    a.partials = s->sweep_part;
+    a.no_xy = (int)s->p.no_xy;
+    a.nzb = (int)s->p.nzb;
+    a.nze = (int)s->p.nze;
+    a.no_z = (int)s->p.no_z;
+    a.n_windows = sh.n_windows;
+    a.wpb = sh.wpb;
+    a.n_kg = sh.n_kg;
+    a.n_jt = sh.n_jt;
+    a.n_ic = sh.n_ic;
+    a.chunk = sh.chunk;
+    a.mid = s->p.mid_xy;
+    a.rsq = s->p.scoring_radius_sq;
+    a.p_lat = s->p.p_lat;
+    a.p_zm = s->p.p_zm;
+    a.p_zp = s->p.p_zp;
+    a.p_cen = s->p.p_cen;
+    a.n_lat = s->p.n_lat;
+    a.n_zm = s->p.n_zm;
+    a.n_zp = s->p.n_zp;
+    a.n_cen = s->p.n_cen;
+    a.alpha_dt = s->p.alpha_dt;
+    const dim3 grid((unsigned)nblk), block((unsigned)(64 * sh.wpb));
+    record(s, 0);
+    int rc = s->prec == PIC_FP64 ? launch_sweep_typed<double, double>(a, sh.tj, grid, block, s->stream, dep)
+                                 : launch_sweep_typed<float, float>(a, sh.tj, grid, block, s->stream, dep);
+    if (rc) return rc;
+    HIP_LAUNCH_CHECK();
+    record(s, 0);
+    return PIC_OK;
+}
+
+int64_t gather_blocks(const pic_sim *s) {
+    const int64_t nt = ceil_div(s->p.no_xy, pic::kGT);
+    return nt * nt;
+}
+
+int enqueue_gather(pic_sim *s, const double *xs, const double *ys, int64_t m) {
+    pic::GatherArgs g{};
+    g.xs = xs;
+    g.ys = ys;
+    g.m = m;
+    g.dens = s->dens;
+    g.partials = s->dep_part;
+    g.no_xy = (int)s->p.no_xy;
+    g.no_z = (int)s->p.no_z;
+    g.ntj = (int)ceil_div(s->p.no_xy, pic::kGT);
+    g.cutoff = s->p.cutoff_voxels;
+    g.h2b2 = s->p.h2_over_b2;
+    g.gf = s->p.gaussian_factor;
+    g.mid = s->p.mid_xy;
+    g.rsq = s->p.scoring_radius_sq;
+    record(s, 1);
+    hipLaunchKernelGGL(pic::gather_kernel, dim3((unsigned)gather_blocks(s)), dim3(256), 0, s->stream, g);
+    HIP_LAUNCH_CHECK();
+    record(s, 1);
+    return PIC_OK;
+}
+
+template <typename S>
+void launch_ring_typed(pic_sim *s, bool dep) {
+    const int64_t n = (4 * s->p.no_xy - 4) * s->p.nzb;
+    const unsigned blocks = (unsigned)std::min<int64_t>(ceil_div(n, 256), 4096);
+    S *np_ = (S *)s->nxt_p, *nn_ = (S *)s->nxt_n;
+    const S *cp_ = (const S *)s->cur_p, *cn_ = (const S *)s->cur_n;
+    const int no_xy = (int)s->p.no_xy, nzb = (int)s->p.nzb, nze = (int)s->p.nze, no_z = (int)s->p.no_z;
+    if (s->p.wall == PIC_WALL_REFLECTING)
+        hipLaunchKernelGGL((pic::ring_kernel<S, true, false>), dim3(blocks), dim3(256), 0, s->stream, np_, nn_, cp_,
+                           cn_, s->dens, no_xy, nzb, nze, no_z);
+    else if (dep)
+        hipLaunchKernelGGL((pic::ring_kernel<S, false, true>), dim3(blocks), dim3(256), 0, s->stream, np_, nn_, cp_,
+                           cn_, s->dens, no_xy, nzb, nze, no_z);
+    else
+        hipLaunchKernelGGL((pic::ring_kernel<S, false, false>), dim3(blocks), dim3(256), 0, s->stream, np_, nn_,
+                           cp_, cn_, s->dens, no_xy, nzb, nze, no_z);
+}
+
+int enqueue_ring(pic_sim *s, bool dep) {
+    if (s->prec == PIC_FP64)
+        launch_ring_typed<double>(s, dep);
+    else
+        launch_ring_typed<float>(s, dep);
+    HIP_LAUNCH_CHECK();
+    return PIC_OK;
+}
+
+int enqueue_reduce(pic_sim *s, bool with_sweep, bool with_dep, double *out4) {
+    const int n_sweep = with_sweep ? (int)sweep_shape(s).blocks() : 0;
+    const int n_dep = with_dep ? (int)gather_blocks(s) : 0;
+    hipLaunchKernelGGL(pic::reduce_kernel, dim3(1), dim3(256), 0, s->stream, s->sweep_part, n_sweep, s->dep_part,
+                       n_dep, out4);
+    HIP_LAUNCH_CHECK();
+    return PIC_OK;
+}
+
+void swap_buffers(pic_sim *s) {
+    std::swap(s->cur_p, s->nxt_p);
+    std::swap(s->cur_n, s->nxt_n);
+}
+
+int check_params(const pic_params *p, int precision) {
+    if (!p) return fail(PIC_E_INVALID, "params is NULL");
+    if (precision != PIC_FP64 && precision != PIC_FP32) return fail(PIC_E_INVALID, "precision must be PIC_FP64 or PIC_FP32");
+    if (p->no_xy < 3 || p->nzb < 3) return fail(PIC_E_INVALID, "grid too small: no_xy=%lld nzb=%lld", (long long)p->no_xy, (long long)p->nzb);
+    if (p->no_xy > 65535 || p->nzb > 65535) return fail(PIC_E_INVALID, "grid axis exceeds 65535");
+    if (p->nze < 0 || p->no_z < 0 || p->nze + p->no_z > p->nzb) return fail(PIC_E_INVALID, "gap layers outside the grid");
+    if (p->wall != PIC_WALL_ABSORBING && p->wall != PIC_WALL_REFLECTING) return fail(PIC_E_INVALID, "wall must be PIC_WALL_ABSORBING or PIC_WALL_REFLECTING");
+    if (!(p->cutoff_voxels > 0.0)) return fail(PIC_E_INVALID, "cutoff_voxels must be positive");
+    return PIC_OK;
+}
+
+size_t partial_cap_bytes(const pic_params *p) {
+    // generous upper bound on sweep blocks: one per (row, window, 8 planes)
+    const int64_t rows = p->no_xy - 2;
+    const int64_t windows = ceil_div(p->nzb - 2, pic::kOut);
+    const int64_t cap = rows * windows * std::max<int64_t>(1, ceil_div(rows, 8)) + 1024;
+    return (size_t)cap * 3 * sizeof(double);
+}
+
+}  // namespace
+
+extern "C" {
+
+int pic_last_error(char *buf, size_t len) {
+    if (!buf || len == 0) return PIC_E_INVALID;
+    snprintf(buf, len, "%s", g_err);
+    return PIC_OK;
+}
+
+int pic_device_info(char *name, size_t name_len, size_t *free_bytes, size_t *total_bytes) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(PIC_E_NODEVICE, "no HIP device is visible");
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, dev));
+    if (name && name_len) snprintf(name, name_len, "%s (%s)", prop.name, prop.gcnArchName);
+    size_t fr = 0, tot = 0;
+    HIP_TRY(hipMemGetInfo(&fr, &tot));
+    if (free_bytes) *free_bytes = fr;
+    if (total_bytes) *total_bytes = tot;
+    return PIC_OK;
+}
+
+int pic_mem_required(const pic_params *p, int precision, size_t *bytes) {
+    int rc = check_params(p, precision);
+    if (rc) return rc;
+    if (!bytes) return fail(PIC_E_INVALID, "bytes is NULL");
+    const size_t elem = precision == PIC_FP64 ? 8 : 4;
+    const size_t vox = (size_t)p->no_xy * p->no_xy * p->nzb;
+    const size_t xy = (size_t)p->no_xy * p->no_xy;
+    const size_t gt = (size_t)ceil_div(p->no_xy, pic::kGT);
+    *bytes = 4 * vox * elem + xy * 8 + partial_cap_bytes(p) + gt * gt * 8 + 64;
+    return PIC_OK;
+}
+
+int pic_create(const pic_params *p, int precision, pic_sim **out) {
+    if (!out) return fail(PIC_E_INVALID, "out is NULL");
+    *out = nullptr;
+    int rc = check_params(p, precision);
+    if (rc) return rc;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(PIC_E_NODEVICE, "no HIP device is visible");
+    size_t need = 0;
+    pic_mem_required(p, precision, &need);
+    size_t fr = 0, tot = 0;
+    HIP_TRY(hipMemGetInfo(&fr, &tot));
+    if ((double)need > 0.9 * (double)fr)
+        return fail(PIC_E_NOMEM, "grid %lldx%lldx%lld needs %.3f GiB of device memory, %.3f GiB free",
+                    (long long)p->no_xy, (long long)p->no_xy, (long long)p->nzb, need / 1073741824.0,
+                    fr / 1073741824.0);
+    pic_sim *s = new pic_sim();
+    s->p = *p;
+    s->prec = precision;
+    s->elem = precision == PIC_FP64 ? 8 : 4;
+    s->n_vox = p->no_xy * p->no_xy * p->nzb;
+    HIP_TRY(hipGetDevice(&s->device));
+    auto cleanup = [&](int code) {
+        pic_destroy(s);
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
+        return cleanup(fail(PIC_E_HIP, "hipStreamCreate failed"));
+    void **bufs[4] = {&s->cur_p, &s->cur_n, &s->nxt_p, &s->nxt_n};
+    for (void **b : bufs) {
+        if (hipMalloc(b, s->n_vox * s->elem) != hipSuccess) return cleanup(fail(PIC_E_NOMEM, "hipMalloc of a carrier array failed"));
+        if (hipMemsetAsync(*b, 0, s->n_vox * s->elem, s->stream) != hipSuccess) return cleanup(fail(PIC_E_HIP, "hipMemsetAsync failed"));
+    }
+    const size_t xy = (size_t)p->no_xy * p->no_xy;
+    s->sweep_part_cap = (int64_t)(partial_cap_bytes(p) / (3 * sizeof(double)));
+    s->dep_part_cap = gather_blocks(s);
+    if (hipMalloc(&s->dens, xy * 8) != hipSuccess || hipMalloc(&s->sweep_part, s->sweep_part_cap * 3 * 8) != hipSuccess ||
+        hipMalloc(&s->dep_part, s->dep_part_cap * 8) != hipSuccess || hipMalloc(&s->scratch4, 8 * 8) != hipSuccess)
+        return cleanup(fail(PIC_E_NOMEM, "hipMalloc of scratch failed"));
+    hipMemsetAsync(s->dens, 0, xy * 8, s->stream);
+    hipMemsetAsync(s->scratch4, 0, 64, s->stream);
+    if (hipStreamSynchronize(s->stream) != hipSuccess) return cleanup(fail(PIC_E_HIP, "initialisation failed"));
+    *out = s;
+    return PIC_OK;
+}
+
+int pic_destroy(pic_sim *s) {
+    if (!s) return PIC_OK;
+    if (s->stream) hipStreamSynchronize(s->stream);
+    void *bufs[] = {s->cur_p, s->cur_n, s->nxt_p, s->nxt_n, s->dens, s->sweep_part, s->dep_part,
+                    s->scratch4, s->per_step, s->xs, s->ys};
+    for (void *b : bufs)
+        if (b) hipFree(b);
+    for (hipEvent_t e : s->ev_pool) hipEventDestroy(e);
+    if (s->stream) hipStreamDestroy(s->stream);
+    delete s;
+    return PIC_OK;
+}
+
+int pic_upload_tracks(pic_sim *s, const double *xs, const double *ys, int64_t n_tracks, const int64_t *offsets,
+                      int64_t n_steps) {
+    if (!s) return fail(PIC_E_INVALID, "sim is NULL");
+    if (n_tracks < 0 || n_steps < 0 || !offsets) return fail(PIC_E_INVALID, "bad track arrays");
+    if (n_tracks > 0 && (!xs || !ys)) return fail(PIC_E_INVALID, "xs/ys NULL with n_tracks > 0");
+    if (offsets[0] != 0 || offsets[n_steps] != n_tracks) return fail(PIC_E_INVALID, "offsets must run from 0 to n_tracks");
+    for (int64_t i = 0; i < n_steps; ++i)
+        if (offsets[i + 1] < offsets[i]) return fail(PIC_E_INVALID, "offsets must be non-decreasing");
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    if (s->xs) hipFree(s->xs);
+    if (s->ys) hipFree(s->ys);
+    if (s->per_step) hipFree(s->per_step);
+    s->xs = s->ys = s->per_step = nullptr;
+    const size_t tb = (size_t)std::max<int64_t>(n_tracks, 1) * 8;
+    if (hipMalloc(&s->xs, tb) != hipSuccess || hipMalloc(&s->ys, tb) != hipSuccess ||
+        hipMalloc(&s->per_step, (size_t)std::max<int64_t>(n_steps, 1) * 4 * 8) != hipSuccess)
+        return fail(PIC_E_NOMEM, "hipMalloc for %lld tracks failed", (long long)n_tracks);
+    if (n_tracks > 0) {
+        HIP_TRY(hipMemcpyAsync(s->xs, xs, n_tracks * 8, hipMemcpyHostToDevice, s->stream));
+        HIP_TRY(hipMemcpyAsync(s->ys, ys, n_tracks * 8, hipMemcpyHostToDevice, s->stream));
+    }
+    HIP_TRY(hipMemsetAsync(s->per_step, 0, (size_t)std::max<int64_t>(n_steps, 1) * 4 * 8, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->offsets.assign(offsets, offsets + n_steps + 1);
+    s->n_tracks = n_tracks;
+    s->n_steps = n_steps;
+    return PIC_OK;
+}
+
+int pic_run_steps(pic_sim *s, int64_t first, int64_t count) {
+    if (!s) return fail(PIC_E_INVALID, "sim is NULL");
+    if (first < 0 || count < 0 || first + count > s->n_steps)
+        return fail(PIC_E_INVALID, "steps [%lld, %lld) outside the uploaded %lld", (long long)first,
+                    (long long)(first + count), (long long)s->n_steps);
+    if (s->timing) {
+        for (hipEvent_t e : s->ev_pool) hipEventDestroy(e);
+        s->ev_pool.clear();
+        s->ev_kind.clear();
+    }
+    for (int64_t step = first; step < first + count; ++step) {
+        const int64_t m = s->offsets[step + 1] - s->offsets[step];
+        const bool dep = m > 0;
+        int rc;
+        if (dep && (rc = enqueue_gather(s, s->xs + s->offsets[step], s->ys + s->offsets[step], m))) return rc;
+        if ((rc = enqueue_sweep(s, dep))) return rc;
+        if ((rc = enqueue_ring(s, dep))) return rc;
+        if ((rc = enqueue_reduce(s, true, dep, s->per_step + 4 * step))) return rc;
+        swap_buffers(s);
+    }
+    return PIC_OK;
+}
+
+int pic_read_per_step(pic_sim *s, int64_t first, int64_t count, double *out) {
+    if (!s || !out) return fail(PIC_E_INVALID, "NULL argument");
+    if (first < 0 || count < 0 || first + count > s->n_steps) return fail(PIC_E_INVALID, "rows outside the record");
+    if (count == 0) return PIC_OK;
+    HIP_TRY(hipMemcpyAsync(out, s->per_step + 4 * first, count * 4 * 8, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return PIC_OK;
+}
+
+int pic_deposit(pic_sim *s, const double *xs, const double *ys, int64_t m, double *inserted) {
+    if (!s) return fail(PIC_E_INVALID, "sim is NULL");
+    if (m < 0 || (m > 0 && (!xs || !ys))) return fail(PIC_E_INVALID, "bad track arrays");
+    double host4[4] = {0, 0, 0, 0};
+    if (m > 0) {
+        double *dx = nullptr, *dy = nullptr;
+        if (hipMalloc(&dx, m * 8) != hipSuccess || hipMalloc(&dy, m * 8) != hipSuccess) {
+            if (dx) hipFree(dx);
+            return fail(PIC_E_NOMEM, "hipMalloc for %lld tracks failed", (long long)m);
+        }
+        int rc = PIC_OK;
+        if (hipMemcpyAsync(dx, xs, m * 8, hipMemcpyHostToDevice, s->stream) != hipSuccess ||
+            hipMemcpyAsync(dy, ys, m * 8, hipMemcpyHostToDevice, s->stream) != hipSuccess)
+            rc = fail(PIC_E_HIP, "track upload failed");
+        if (!rc) rc = enqueue_gather(s, dx, dy, m);
+        if (!rc) {
+            const int64_t n = s->p.no_xy * s->p.no_xy * s->p.no_z;
+            const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, 256), 8192));
+            const int no_xy = (int)s->p.no_xy, nzb = (int)s->p.nzb, nze = (int)s->p.nze, no_z = (int)s->p.no_z;
+            if (s->prec == PIC_FP64)
+                hipLaunchKernelGGL(pic::broadcast_kernel<double>, dim3(blocks), dim3(256), 0, s->stream,
+                                   (double *)s->cur_p, (double *)s->cur_n, s->dens, no_xy, nzb, nze, no_z);
+            else
+                hipLaunchKernelGGL(pic::broadcast_kernel<float>, dim3(blocks), dim3(256), 0, s->stream,
+                                   (float *)s->cur_p, (float *)s->cur_n, s->dens, no_xy, nzb, nze, no_z);
+            if (hipGetLastError() != hipSuccess) rc = fail(PIC_E_HIP, "broadcast launch failed");
+        }
+        if (!rc) rc = enqueue_reduce(s, false, true, s->scratch4);
+        if (!rc && (hipMemcpyAsync(host4, s->scratch4, 32, hipMemcpyDeviceToHost, s->stream) != hipSuccess ||
+                    hipStreamSynchronize(s->stream) != hipSuccess))
+            rc = fail(PIC_E_HIP, "deposit readback failed");
+        hipStreamSynchronize(s->stream);
+        hipFree(dx);
+        hipFree(dy);
+        if (rc) return rc;
+    }
+    if (inserted) *inserted = host4[0];
+    return PIC_OK;
+}
+
+int pic_step(pic_sim *s, double out3[3]) {
+    if (!s) return fail(PIC_E_INVALID, "sim is NULL");
+    int rc;
+    if ((rc = enqueue_sweep(s, false))) return rc;
+    if ((rc = enqueue_ring(s, false))) return rc;
+    if ((rc = enqueue_reduce(s, true, false, s->scratch4))) return rc;
+    swap_buffers(s);
+    double host4[4];
+    HIP_TRY(hipMemcpyAsync(host4, s->scratch4, 32, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    if (out3) {
+        out3[0] = host4[1];
+        out3[1] = host4[2];
+        out3[2] = host4[3];
+    }
+    return PIC_OK;
+}
+
+int pic_read_fields(pic_sim *s, double *pos, double *neg) {
+    if (!s) return fail(PIC_E_INVALID, "sim is NULL");
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    void *src[2] = {s->cur_p, s->cur_n};
+    double *dst[2] = {pos, neg};
+    std::vector<float> tmp;
+    for (int q = 0; q < 2; ++q) {
+        if (!dst[q]) continue;
+        if (s->prec == PIC_FP64) {
+            HIP_TRY(hipMemcpy(dst[q], src[q], s->n_vox * 8, hipMemcpyDeviceToHost));
+        } else {
+            tmp.resize(s->n_vox);
+            HIP_TRY(hipMemcpy(tmp.data(), src[q], s->n_vox * 4, hipMemcpyDeviceToHost));
+            for (int64_t e = 0; e < s->n_vox; ++e) dst[q][e] = (double)tmp[e];
+        }
+    }
+    return PIC_OK;
+}
+
+int pic_write_fields(pic_sim *s, const double *pos, const double *neg) {
+    if (!s || !pos || !neg) return fail(PIC_E_INVALID, "NULL argument");
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    const double *src[2] = {pos, neg};
+    void *dst[2][2] = {{s->cur_p, s->nxt_p}, {s->cur_n, s->nxt_n}};
+    std::vector<float> tmp;
+    for (int q = 0; q < 2; ++q) {
+        for (int b = 0; b < 2; ++b) {
+            if (s->prec == PIC_FP64) {
+                HIP_TRY(hipMemcpy(dst[q][b], src[q], s->n_vox * 8, hipMemcpyHostToDevice));
+            } else {
+                tmp.resize(s->n_vox);
+                for (int64_t e = 0; e < s->n_vox; ++e) tmp[e] = (float)src[q][e];
+                HIP_TRY(hipMemcpy(dst[q][b], tmp.data(), s->n_vox * 4, hipMemcpyHostToDevice));
+            }
+        }
+    }
+    return PIC_OK;
+}
+
+int pic_synchronize(pic_sim *s) {
+    if (!s) return fail(PIC_E_INVALID, "sim is NULL");
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return PIC_OK;
+}
+
+int pic_set_tiling(pic_sim *s, int rows_per_tile, int planes_per_chunk) {
+    if (!s) return fail(PIC_E_INVALID, "sim is NULL");
+    if (rows_per_tile < 0 || planes_per_chunk < 0) return fail(PIC_E_INVALID, "negative tiling");
+    if (rows_per_tile && rows_per_tile != 1 && rows_per_tile != 2 && rows_per_tile != 4 && rows_per_tile != 8 &&
+        rows_per_tile != 12)
+        return fail(PIC_E_INVALID, "rows_per_tile must be 0, 1, 2, 4, 8 or 12");
+    if (rows_per_tile == 12 && s->prec == PIC_FP64)
+        return fail(PIC_E_INVALID, "rows_per_tile 12 is FP32 only (FP64 would spill registers)");
+    s->force_tj = rows_per_tile;
+    s->force_chunk = planes_per_chunk;
+    const SweepShape sh = sweep_shape(s);
+    if (sh.blocks() > s->sweep_part_cap) {
+        s->force_tj = s->force_chunk = 0;
+        return fail(PIC_E_INVALID, "tiling needs %lld blocks, more than the partial buffer holds", (long long)sh.blocks());
+    }
+    return PIC_OK;
+}
+
+int pic_enable_timing(pic_sim *s, int enable) {
+    if (!s) return fail(PIC_E_INVALID, "sim is NULL");
+    s->timing = enable != 0;
+    return PIC_OK;
+}
+
+int pic_get_timing(pic_sim *s, double *sweep_ms, int64_t *sweep_launches, double *deposit_ms, int64_t *deposit_launches,
+                   double *other_ms) {
+    if (!s) return fail(PIC_E_INVALID, "sim is NULL");
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    double ts = 0, td = 0;
+    int64_t ns = 0, nd = 0;
+    for (size_t q = 0; q + 1 < s->ev_pool.size(); q += 2) {
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, s->ev_pool[q], s->ev_pool[q + 1]));
+        if (s->ev_kind[q] == 0) {
+            ts += ms;
+            ++ns;
+        } else {
+            td += ms;
+            ++nd;
+        }
+    }
+    if (sweep_ms) *sweep_ms = ts;
+    if (sweep_launches) *sweep_launches = ns;
+    if (deposit_ms) *deposit_ms = td;
+    if (deposit_launches) *deposit_launches = nd;
+    if (other_ms) *other_ms = 0.0;
+    return PIC_OK;
+}
+
+}  // extern "C"
