@@ -113,9 +113,10 @@ int pic_write_fields(pic_sim *sim, const double *pos, const double *neg);
 /* Block until all work queued on the handle's stream is done. */
 int pic_synchronize(pic_sim *sim);
 
-/* Sweep launch shape, for tuning: rows per tile and planes per block chunk
- * (0 = automatic). Takes effect at the next launch. */
-int pic_set_tiling(pic_sim *sim, int rows_per_tile, int planes_per_chunk);
+/* Sweep launch shape, for tuning: rows per tile (1, 2, 4, 8), planes per
+ * block chunk, and k values per lane (1, or 2 for FP32 fields with even nzb);
+ * 0 = automatic. Takes effect at the next launch. */
+int pic_set_tiling(pic_sim *sim, int rows_per_tile, int planes_per_chunk, int values_per_lane);
 
 /* Kernel timing of the last pic_run_steps call when timing is enabled:
  * summed milliseconds of the sweep, deposition and wall/reduce kernels

@@ -43,7 +43,7 @@ int fail(int code, const char *fmt, ...) {
 int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 struct SweepShape {
-    int tj = 0, chunk = 0, n_windows = 0, wpb = 0, n_kg = 0, n_jt = 0, n_ic = 0;
+    int tj = 0, chunk = 0, n_windows = 0, wpb = 0, n_kg = 0, n_jt = 0, n_ic = 0, v = 1;
     int64_t blocks() const { return (int64_t)n_kg * n_jt * n_ic; }
 };
 
@@ -68,7 +68,7 @@ struct pic_sim {
     double *xs = nullptr, *ys = nullptr;
     int64_t n_tracks = 0;
     std::vector<int64_t> offsets;
-    int force_tj = 0, force_chunk = 0;
+    int force_tj = 0, force_chunk = 0, force_v1 = 0;
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
     std::vector<int> ev_kind;  // 0 sweep, 1 deposit, 2 other
@@ -89,7 +89,10 @@ int sm_count(int device) {
 SweepShape sweep_shape(const pic_sim *s) {
     SweepShape sh;
     const int64_t no_xy = s->p.no_xy, nzb = s->p.nzb;
-    sh.n_windows = (int)ceil_div(nzb - 2, pic::kOut);
+    // FP32 fields take two k values per lane (8-byte loads) when rows start
+    // 8-byte aligned, i.e. when nzb is even
+    sh.v = (s->prec == PIC_FP32 && nzb % 2 == 0 && !s->force_v1) ? 2 : 1;
+    sh.n_windows = (int)ceil_div(nzb - 2, pic::kWave * sh.v - 2);
     sh.wpb = std::min(sh.n_windows, 8);
     sh.n_kg = (int)ceil_div(sh.n_windows, sh.wpb);
     const int64_t rows = no_xy - 2, planes = no_xy - 2;
@@ -98,11 +101,10 @@ SweepShape sweep_shape(const pic_sim *s) {
     if (tj <= 0) {
         // target ~16 waves per CU in flight; big grids get tall tiles
         const int64_t waves_wanted = (int64_t)cus * 16;
-        // register budget: FP32 tiles up to 12 rows, FP64 up to 8, without spills
-        const int cand[] = {12, 8, 4, 2, 1};
+        const int cand[] = {8, 4, 2, 1};
         tj = 1;
         for (int c : cand) {
-            if (s->prec == PIC_FP64 && c > 8) continue;
+            if (sh.v == 2 && c > 4) continue;  // 8 rows of two values per lane would spill
             const int64_t waves = ceil_div(rows, c) * sh.n_windows * std::max<int64_t>(1, planes / 16);
             if (waves >= waves_wanted) {
                 tj = c;
@@ -124,29 +126,28 @@ SweepShape sweep_shape(const pic_sim *s) {
     return sh;
 }
 
-template <typename S, typename A, int TJ>
+template <typename S, typename A, int TJ, int V>
 void launch_sweep_tj(const pic::SweepArgs &a, dim3 grid, dim3 block, hipStream_t st, bool dep) {
     if (dep)
-        hipLaunchKernelGGL((pic::sweep_kernel<S, A, TJ, true>), grid, block, 0, st, a);
+        hipLaunchKernelGGL((pic::sweep_kernel<S, A, TJ, true, V>), grid, block, 0, st, a);
     else
-        hipLaunchKernelGGL((pic::sweep_kernel<S, A, TJ, false>), grid, block, 0, st, a);
+        hipLaunchKernelGGL((pic::sweep_kernel<S, A, TJ, false, V>), grid, block, 0, st, a);
 }
 
-template <typename S, typename A>
+template <typename S, typename A, int V>
 int launch_sweep_typed(const pic::SweepArgs &a, int tj, dim3 grid, dim3 block, hipStream_t st, bool dep) {
     switch (tj) {
-        case 1: launch_sweep_tj<S, A, 1>(a, grid, block, st, dep); break;
-        case 2: launch_sweep_tj<S, A, 2>(a, grid, block, st, dep); break;
-        case 4: launch_sweep_tj<S, A, 4>(a, grid, block, st, dep); break;
-        case 8: launch_sweep_tj<S, A, 8>(a, grid, block, st, dep); break;
-        case 12:
-            if constexpr (sizeof(S) == 8) {
-                return fail(PIC_E_INVALID, "rows_per_tile 12 is FP32 only");
+        case 1: launch_sweep_tj<S, A, 1, V>(a, grid, block, st, dep); break;
+        case 2: launch_sweep_tj<S, A, 2, V>(a, grid, block, st, dep); break;
+        case 4: launch_sweep_tj<S, A, 4, V>(a, grid, block, st, dep); break;
+        case 8:
+            if constexpr (V == 2) {
+                return fail(PIC_E_INVALID, "rows_per_tile 8 needs one value per lane");
             } else {
-                launch_sweep_tj<S, A, 12>(a, grid, block, st, dep);
+                launch_sweep_tj<S, A, 8, V>(a, grid, block, st, dep);
             }
             break;
-        default: return fail(PIC_E_INVALID, "unsupported rows_per_tile %d (1, 2, 4, 8 or 12)", tj);
+        default: return fail(PIC_E_INVALID, "unsupported rows_per_tile %d (1, 2, 4 or 8)", tj);
     }
     return PIC_OK;
 }
@@ -195,8 +196,9 @@ int enqueue_sweep(pic_sim *s, bool dep) {
     a.alpha_dt = s->p.alpha_dt;
     const dim3 grid((unsigned)nblk), block((unsigned)(64 * sh.wpb));
     record(s, 0);
-    int rc = s->prec == PIC_FP64 ? launch_sweep_typed<double, double>(a, sh.tj, grid, block, s->stream, dep)
-                                 : launch_sweep_typed<float, float>(a, sh.tj, grid, block, s->stream, dep);
+    int rc = s->prec == PIC_FP64 ? launch_sweep_typed<double, double, 1>(a, sh.tj, grid, block, s->stream, dep)
+             : sh.v == 2         ? launch_sweep_typed<float, float, 2>(a, sh.tj, grid, block, s->stream, dep)
+                                 : launch_sweep_typed<float, float, 1>(a, sh.tj, grid, block, s->stream, dep);
     if (rc) return rc;
     HIP_LAUNCH_CHECK();
     record(s, 0);
@@ -549,19 +551,20 @@ int pic_synchronize(pic_sim *s) {
     return PIC_OK;
 }
 
-int pic_set_tiling(pic_sim *s, int rows_per_tile, int planes_per_chunk) {
+int pic_set_tiling(pic_sim *s, int rows_per_tile, int planes_per_chunk, int values_per_lane) {
     if (!s) return fail(PIC_E_INVALID, "sim is NULL");
     if (rows_per_tile < 0 || planes_per_chunk < 0) return fail(PIC_E_INVALID, "negative tiling");
-    if (rows_per_tile && rows_per_tile != 1 && rows_per_tile != 2 && rows_per_tile != 4 && rows_per_tile != 8 &&
-        rows_per_tile != 12)
-        return fail(PIC_E_INVALID, "rows_per_tile must be 0, 1, 2, 4, 8 or 12");
-    if (rows_per_tile == 12 && s->prec == PIC_FP64)
-        return fail(PIC_E_INVALID, "rows_per_tile 12 is FP32 only (FP64 would spill registers)");
+    if (values_per_lane < 0 || values_per_lane > 2) return fail(PIC_E_INVALID, "values_per_lane must be 0, 1 or 2");
+    if (values_per_lane == 2 && (s->prec != PIC_FP32 || s->p.nzb % 2))
+        return fail(PIC_E_INVALID, "two values per lane needs FP32 fields and an even nzb");
+    s->force_v1 = values_per_lane == 1;
+    if (rows_per_tile && rows_per_tile != 1 && rows_per_tile != 2 && rows_per_tile != 4 && rows_per_tile != 8)
+        return fail(PIC_E_INVALID, "rows_per_tile must be 0, 1, 2, 4 or 8");
     s->force_tj = rows_per_tile;
     s->force_chunk = planes_per_chunk;
     const SweepShape sh = sweep_shape(s);
     if (sh.blocks() > s->sweep_part_cap) {
-        s->force_tj = s->force_chunk = 0;
+        s->force_tj = s->force_chunk = s->force_v1 = 0;
         return fail(PIC_E_INVALID, "tiling needs %lld blocks, more than the partial buffer holds", (long long)sh.blocks());
     }
     return PIC_OK;

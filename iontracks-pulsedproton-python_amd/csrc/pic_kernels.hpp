@@ -72,26 +72,35 @@ struct SweepArgs {
     double alpha_dt;
 };
 
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+
 // One Lax-Wendroff step of both species, fused with the gap-layer deposition
 // of this step (DEP) and the scored reduction.
 //
-// Work decomposition: a wave owns a k window (lanes 0..63 <-> k = 62w+lane,
-// lanes 1..62 write), TJ consecutive interior rows j of one tile, and marches
-// through a chunk of i planes. Per lane it keeps, in registers, the tile's rows
-// of plane i-1, plane i (with the two halo rows j0-1 and j0+TJ) and plane i+1,
-// so the j and i neighbours are register reads and every value is fetched from
-// memory once per plane (plus the two halo rows). The k neighbours come from
-// the adjacent lanes by wave shuffles; the window overlap of two lanes supplies
-// the k neighbours at the window edges. No LDS staging and no barriers in the
-// march: waves are independent.
+// Work decomposition: a wave owns a k window, TJ consecutive interior rows j of
+// one tile, and marches through a chunk of i planes. Each lane holds V
+// consecutive k values (V = 2 for FP32 fields, one 8-byte load per row, so a
+// wave moves 512 B per load as the FP64 path does with V = 1); a window of
+// 64*V values writes all but its first and last one, and consecutive windows
+// overlap by two so those edge values are supplied by the neighbour window.
+// Per lane it keeps, in registers, the tile's rows of plane i-1, plane i (with
+// the two halo rows j0-1 and j0+TJ), plane i+1, and plane i+2 which is already
+// being loaded while plane i is computed, so memory latency overlaps the
+// arithmetic of a whole plane. The j and i neighbours are register reads and
+// every value is fetched from memory once per plane (plus the halo rows, which
+// the neighbouring tile on the same XCD has usually just brought into L2). The
+// k neighbours come from the adjacent lanes by wave shuffles. No LDS staging
+// and no barriers in the march: waves are independent.
 //
 // The arithmetic is the reference's, term for term and in the same order
 // (solver_numba_parallel.py:374-405); with S = A = double and contraction off
 // a voxel update is bit-identical to the reference's FP64 sweep. With
 // S = A = float the stencil runs in FP32 while the recombination product, the
 // deposited density and every scored sum stay FP64.
-template <typename S, typename A, int TJ, bool DEP>
+template <typename S, typename A, int TJ, bool DEP, int V>
 __global__ void __launch_bounds__(512) sweep_kernel(SweepArgs a) {
+    static_assert(V == 1 || (V == 2 && sizeof(S) == 4), "two values per lane is the FP32 layout");
+    constexpr int WIN = kWave * V - 2;  // outputs per window
     const int logical = xcd_logical_block(blockIdx.x, gridDim.x);
     const int kg = logical % a.n_kg;
     const int rest = logical / a.n_kg;
@@ -103,10 +112,15 @@ __global__ void __launch_bounds__(512) sweep_kernel(SweepArgs a) {
     double acc_r = 0.0, acc_p = 0.0, acc_n = 0.0;
     if (win < a.n_windows) {
         const int no_xy = a.no_xy, nzb = a.nzb;
-        const int k = win * kOut + lane;
-        const bool k_in = k < nzb;
-        const bool k_out = lane >= 1 && lane <= kOut && k <= nzb - 2;
-        const bool k_gap = k >= a.nze && k < a.nze + a.no_z;
+        const int k0 = win * WIN + V * lane;
+        const bool k_in = k0 < nzb;  // with V = 2, nzb is even so k0+1 < nzb too
+        bool k_out[V], k_gap[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            const int q = V * lane + v, k = k0 + v;
+            k_out[v] = q >= 1 && q <= kWave * V - 2 && k >= 1 && k <= nzb - 2;
+            k_gap[v] = k >= a.nze && k < a.nze + a.no_z;
+        }
         const int j0 = 1 + jt * TJ;
         const int nrows = min(TJ, no_xy - 1 - j0);  // valid own rows
         const int i0 = 1 + ic * a.chunk;
@@ -117,91 +131,144 @@ __global__ void __launch_bounds__(512) sweep_kernel(SweepArgs a) {
         S *NO = static_cast<S *>(a.neg_out);
         const int64_t plane = (int64_t)no_xy * nzb;
 
-        // value of (i, j, k) with this step's deposition added in the gap layers;
-        // the row base is wave-uniform, so each load is one scalar base plus
-        // the lane's k offset
-        auto load2 = [&](int i, int j, A &vp, A &vn) {
+        // values of (i, j, k0..k0+V-1) with this step's deposition added in the
+        // gap layers; the row base is wave-uniform, so each load is one scalar
+        // base plus the lane's k offset
+        auto load2 = [&](int i, int j, A (&vp)[V], A (&vn)[V]) {
             const int64_t row = (int64_t)i * plane + (int64_t)j * nzb;
-            const S *rp = P + row;
-            const S *rn = N + row;
-            S sp = (S)0, sn = (S)0;
+            S sp[V], sn[V];
+#pragma unroll
+            for (int v = 0; v < V; ++v) sp[v] = sn[v] = (S)0;
             if (k_in) {
-                sp = rp[k];
-                sn = rn[k];
-            }
-            if (DEP) {
-                const double d = a.dens[(int64_t)i * no_xy + j];
-                if (k_gap) {
-                    vp = (A)((double)sp + d);
-                    vn = (A)((double)sn + d);
-                    return;
+                if constexpr (V == 2) {
+                    const f32x2 tp = *reinterpret_cast<const f32x2 *>(P + row + k0);
+                    const f32x2 tn = *reinterpret_cast<const f32x2 *>(N + row + k0);
+                    sp[0] = tp.x;
+                    sp[1] = tp.y;
+                    sn[0] = tn.x;
+                    sn[1] = tn.y;
+                } else {
+                    sp[0] = (P + row)[k0];
+                    sn[0] = (N + row)[k0];
                 }
             }
-            vp = (A)sp;
-            vn = (A)sn;
+            double d = 0.0;
+            if (DEP) d = a.dens[(int64_t)i * no_xy + j];
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                if (DEP && k_gap[v]) {
+                    vp[v] = (A)((double)sp[v] + d);
+                    vn[v] = (A)((double)sn[v] + d);
+                } else {
+                    vp[v] = (A)sp[v];
+                    vn[v] = (A)sn[v];
+                }
+            }
         };
-
-        A pp[TJ], pn[TJ];          // plane i-1, own rows
-        A cp[TJ + 2], cn[TJ + 2];  // plane i, rows j0-1 .. j0+TJ
-        A xp[TJ + 2], xn[TJ + 2];  // plane i+1
-#pragma unroll
-        for (int r = 0; r < TJ; ++r) {
-            pp[r] = pn[r] = (A)0;
-            if (r < nrows) load2(i0 - 1, j0 + r, pp[r], pn[r]);
-        }
-#pragma unroll
-        for (int r = 0; r < TJ + 2; ++r) {
-            cp[r] = cn[r] = (A)0;
-            if (r <= nrows + 1) load2(i0, j0 + r - 1, cp[r], cn[r]);
-        }
-
-        for (int i = i0; i < i1; ++i) {
+        auto load_rows = [&](int i, A (&rp)[TJ + 2][V], A (&rn)[TJ + 2][V]) {
 #pragma unroll
             for (int r = 0; r < TJ + 2; ++r) {
-                xp[r] = xn[r] = (A)0;
-                if (r <= nrows + 1) load2(i + 1, j0 + r - 1, xp[r], xn[r]);
+#pragma unroll
+                for (int v = 0; v < V; ++v) rp[r][v] = rn[r][v] = (A)0;
+                if (r <= nrows + 1) load2(i, j0 + r - 1, rp[r], rn[r]);
             }
+        };
+
+        A pp[TJ][V], pn[TJ][V];          // plane i-1, own rows
+        A cp[TJ + 2][V], cn[TJ + 2][V];  // plane i, rows j0-1 .. j0+TJ
+        A xp[TJ + 2][V], xn[TJ + 2][V];  // plane i+1
+        A fp[TJ + 2][V], fn[TJ + 2][V];  // plane i+2, in flight
+#pragma unroll
+        for (int r = 0; r < TJ; ++r) {
+#pragma unroll
+            for (int v = 0; v < V; ++v) pp[r][v] = pn[r][v] = (A)0;
+            if (r < nrows) load2(i0 - 1, j0 + r, pp[r], pn[r]);
+        }
+        load_rows(i0, cp, cn);
+        load_rows(i0 + 1, xp, xn);
+
+        for (int i = i0; i < i1; ++i) {
+            if (i + 1 < i1) load_rows(i + 2, fp, fn);
             const double di = (double)i - a.mid;
             const double di2 = di * di;
 #pragma unroll
             for (int r = 0; r < TJ; ++r) {
-                const A p = cp[r + 1], n = cn[r + 1];
-                const A p_km = __shfl_up(p, 1, kWave), p_kp = __shfl_down(p, 1, kWave);
-                const A n_km = __shfl_up(n, 1, kWave), n_kp = __shfl_down(n, 1, kWave);
+                A p_km[V], p_kp[V], n_km[V], n_kp[V];
+                if constexpr (V == 2) {
+                    p_km[0] = __shfl_up(cp[r + 1][1], 1, kWave);
+                    p_kp[0] = cp[r + 1][1];
+                    p_km[1] = cp[r + 1][0];
+                    p_kp[1] = __shfl_down(cp[r + 1][0], 1, kWave);
+                    n_km[0] = __shfl_up(cn[r + 1][1], 1, kWave);
+                    n_kp[0] = cn[r + 1][1];
+                    n_km[1] = cn[r + 1][0];
+                    n_kp[1] = __shfl_down(cn[r + 1][0], 1, kWave);
+                } else {
+                    p_km[0] = __shfl_up(cp[r + 1][0], 1, kWave);
+                    p_kp[0] = __shfl_down(cp[r + 1][0], 1, kWave);
+                    n_km[0] = __shfl_up(cn[r + 1][0], 1, kWave);
+                    n_kp[0] = __shfl_down(cn[r + 1][0], 1, kWave);
+                }
                 if (r < nrows) {
-                    const A p_new = (A)a.p_zm * p_km + (A)a.p_zp * p_kp + (A)a.p_lat * cp[r] +
-                                    (A)a.p_lat * cp[r + 2] + (A)a.p_lat * pp[r] + (A)a.p_lat * xp[r + 1] +
-                                    (A)a.p_cen * p;
-                    const A n_new = (A)a.n_zm * n_kp + (A)a.n_zp * n_km + (A)a.n_lat * cn[r] +
-                                    (A)a.n_lat * cn[r + 2] + (A)a.n_lat * pn[r] + (A)a.n_lat * xn[r + 1] +
-                                    (A)a.n_cen * n;
-                    const double rec = a.alpha_dt * (double)p * (double)n;
-                    const S po = (S)((double)p_new - rec);
-                    const S no = (S)((double)n_new - rec);
-                    if (k_out) {
-                        const int j = j0 + r;
-                        const int64_t row = (int64_t)i * plane + (int64_t)j * nzb;
-                        (PO + row)[k] = po;
-                        (NO + row)[k] = no;
-                        const double dj = (double)j - a.mid;
-                        if (k_gap && di2 + dj * dj < a.rsq) {
+                    const int j = j0 + r;
+                    const double dj = (double)j - a.mid;
+                    const bool inside = di2 + dj * dj < a.rsq;
+                    S po[V], no[V];
+#pragma unroll
+                    for (int v = 0; v < V; ++v) {
+                        const A p = cp[r + 1][v], n = cn[r + 1][v];
+                        const A p_new = (A)a.p_zm * p_km[v] + (A)a.p_zp * p_kp[v] + (A)a.p_lat * cp[r][v] +
+                                        (A)a.p_lat * cp[r + 2][v] + (A)a.p_lat * pp[r][v] +
+                                        (A)a.p_lat * xp[r + 1][v] + (A)a.p_cen * p;
+                        const A n_new = (A)a.n_zm * n_kp[v] + (A)a.n_zp * n_km[v] + (A)a.n_lat * cn[r][v] +
+                                        (A)a.n_lat * cn[r + 2][v] + (A)a.n_lat * pn[r][v] +
+                                        (A)a.n_lat * xn[r + 1][v] + (A)a.n_cen * n;
+                        const double rec = a.alpha_dt * (double)p * (double)n;
+                        po[v] = (S)((double)p_new - rec);
+                        no[v] = (S)((double)n_new - rec);
+                        if (k_out[v] && k_gap[v] && inside) {
                             acc_r += rec;
-                            acc_p += (double)po;
-                            acc_n += (double)no;
+                            acc_p += (double)po[v];
+                            acc_n += (double)no[v];
+                        }
+                    }
+                    const int64_t row = (int64_t)i * plane + (int64_t)j * nzb;
+                    if constexpr (V == 2) {
+                        if (k_out[0] && k_out[1]) {
+                            *reinterpret_cast<f32x2 *>(PO + row + k0) = f32x2{po[0], po[1]};
+                            *reinterpret_cast<f32x2 *>(NO + row + k0) = f32x2{no[0], no[1]};
+                        } else {
+#pragma unroll
+                            for (int v = 0; v < V; ++v)
+                                if (k_out[v]) {
+                                    (PO + row)[k0 + v] = po[v];
+                                    (NO + row)[k0 + v] = no[v];
+                                }
+                        }
+                    } else {
+                        if (k_out[0]) {
+                            (PO + row)[k0] = po[0];
+                            (NO + row)[k0] = no[0];
                         }
                     }
                 }
             }
 #pragma unroll
-            for (int r = 0; r < TJ; ++r) {
-                pp[r] = cp[r + 1];
-                pn[r] = cn[r + 1];
-            }
+            for (int r = 0; r < TJ; ++r)
 #pragma unroll
-            for (int r = 0; r < TJ + 2; ++r) {
-                cp[r] = xp[r];
-                cn[r] = xn[r];
-            }
+                for (int v = 0; v < V; ++v) {
+                    pp[r][v] = cp[r + 1][v];
+                    pn[r][v] = cn[r + 1][v];
+                }
+#pragma unroll
+            for (int r = 0; r < TJ + 2; ++r)
+#pragma unroll
+                for (int v = 0; v < V; ++v) {
+                    cp[r][v] = xp[r][v];
+                    cn[r][v] = xn[r][v];
+                    xp[r][v] = fp[r][v];
+                    xn[r][v] = fn[r][v];
+                }
         }
     }
     block_sum3(acc_r, acc_p, acc_n);

@@ -112,7 +112,7 @@ def open_library():
         "pic_read_fields": [_VP, _DP, _DP],
         "pic_write_fields": [_VP, _DP, _DP],
         "pic_synchronize": [_VP],
-        "pic_set_tiling": [_VP, ctypes.c_int, ctypes.c_int],
+        "pic_set_tiling": [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int],
         "pic_enable_timing": [_VP, ctypes.c_int],
         "pic_get_timing": [_VP, _DP, _IP, _DP, _IP, _DP],
     }

@@ -111,8 +111,9 @@ class HipSimulation:
     def synchronize(self) -> None:
         _lib.check(self.L.pic_synchronize(self.handle), "pic_synchronize")
 
-    def set_tiling(self, rows_per_tile: int = 0, planes_per_chunk: int = 0) -> None:
-        _lib.check(self.L.pic_set_tiling(self.handle, int(rows_per_tile), int(planes_per_chunk)), "pic_set_tiling")
+    def set_tiling(self, rows_per_tile: int = 0, planes_per_chunk: int = 0, values_per_lane: int = 0) -> None:
+        _lib.check(self.L.pic_set_tiling(self.handle, int(rows_per_tile), int(planes_per_chunk), int(values_per_lane)),
+                   "pic_set_tiling")
 
     def enable_timing(self, on: bool = True) -> None:
         _lib.check(self.L.pic_enable_timing(self.handle, 1 if on else 0), "pic_enable_timing")
