@@ -110,6 +110,10 @@ int pic_read_fields(pic_sim *sim, double *pos, double *neg);
 /* Overwrite the current (and next) arrays from host FP64 data (test hook). */
 int pic_write_fields(pic_sim *sim, const double *pos, const double *neg);
 
+/* Zero the four carrier arrays and the per-step record, keeping the uploaded
+ * tracks: the state of a fresh handle (a rerun of the same schedule). */
+int pic_reset(pic_sim *sim);
+
 /* Block until all work queued on the handle's stream is done. */
 int pic_synchronize(pic_sim *sim);
 
@@ -119,8 +123,9 @@ int pic_synchronize(pic_sim *sim);
 int pic_set_tiling(pic_sim *sim, int rows_per_tile, int planes_per_chunk, int values_per_lane);
 
 /* Kernel timing of the last pic_run_steps call when timing is enabled:
- * summed milliseconds of the sweep, deposition and wall/reduce kernels
- * measured with HIP events on the handle's stream, and launch counts. */
+ * summed milliseconds of the sweep and of the deposition kernels (with their
+ * launch counts) and of the ring + reduce kernels, measured with HIP events
+ * recorded around each launch on the handle's stream. */
 int pic_enable_timing(pic_sim *sim, int enable);
 int pic_get_timing(pic_sim *sim, double *sweep_ms, int64_t *sweep_launches, double *deposit_ms,
                    int64_t *deposit_launches, double *other_ms);

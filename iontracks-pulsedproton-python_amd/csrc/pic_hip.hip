@@ -2,9 +2,11 @@
 //
 // One handle = one HIP stream + the four carrier arrays (current and next of
 // both species) + scratch. The time-step loop runs here, on the host side of
-// the library, enqueuing per step: gather deposition (in-pulse steps only),
-// the fused sweep, the ring update, and the per-step reduction into a device
-// (n_steps, 4) record. Nothing is copied back per step.
+// the library: for each batch of up to `batch` steps one gather launch
+// computes the deposition density of every in-pulse step of the batch, then
+// one sweep launch per step does deposition-add, Lax-Wendroff update, lateral
+// ring and the per-step reduction into a device (n_steps, 4) record. Nothing
+// is copied back per step.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -57,11 +59,20 @@ struct pic_sim {
     size_t elem = 8;
     int64_t n_vox = 0;
     void *cur_p = nullptr, *cur_n = nullptr, *nxt_p = nullptr, *nxt_n = nullptr;
-    double *dens = nullptr;
-    double *sweep_part = nullptr;
+    double *dens = nullptr;        // [batch + 1 slots][no_xy^2]
+    double *sweep_part = nullptr;  // [2 slots][sweep_part_cap][3]
     int64_t sweep_part_cap = 0;
-    double *dep_part = nullptr;
-    int64_t dep_part_cap = 0;
+    double *dep_part = nullptr;    // [batch + 1 slots][gather tiles]
+    int64_t batch = 1;             // steps per gather launch
+    int64_t *offsets_d = nullptr;
+    // a step whose partials are written but not yet folded into its record
+    struct Pending {
+        const double *part = nullptr;
+        int n = 0;
+        const double *dep = nullptr;
+        int n_dep = 0;
+        double *out4 = nullptr;
+    } pending;
     double *scratch4 = nullptr;  // 8 doubles for the kernel-level calls
     double *per_step = nullptr;
     int64_t n_steps = 0;
@@ -70,10 +81,10 @@ struct pic_sim {
     std::vector<int64_t> offsets;
     int force_tj = 0, force_chunk = 0, force_v1 = 0;
     bool timing = false;
-    std::vector<hipEvent_t> ev_pool;
-    std::vector<int> ev_kind;  // 0 sweep, 1 deposit, 2 other
-    double t_sweep = 0, t_dep = 0, t_other = 0;
-    int64_t n_sweep = 0, n_dep = 0;
+    std::vector<hipEvent_t> ev_pool;  // reused across calls
+    std::vector<int> ev_kind;         // per recorded event: 0 sweep, 1 deposit, 2 ring + reduce
+    size_t ev_used = 0;
+    bool in_run = false;  // events are recorded inside pic_run_steps only
 };
 
 namespace {
@@ -97,54 +108,63 @@ SweepShape sweep_shape(const pic_sim *s) {
     sh.n_kg = (int)ceil_div(sh.n_windows, sh.wpb);
     const int64_t rows = no_xy - 2, planes = no_xy - 2;
     const int cus = sm_count(s->device);
+    // Enough waves for ~16 per CU, with the tallest tile (fewest halo rows) and
+    // the longest chunk (fewest prologue planes) that still get there. Measured
+    // on MI355X (in-pulse steps): FP32 fields best at 4-row tiles on the
+    // 1066^2 x 410 grid and 2-row tiles on 208^2 x 406; FP64 fields at 2-row
+    // tiles on both (4 rows costs occupancy); 16 planes per block throughout.
+    const int64_t waves_wanted = (int64_t)cus * 16;
     int tj = s->force_tj;
+    int chunk = s->force_chunk;
+    const int max_tj = s->prec == PIC_FP64 ? 2 : (sh.v == 1 ? 8 : 4);
     if (tj <= 0) {
-        // target ~16 waves per CU in flight; big grids get tall tiles
-        const int64_t waves_wanted = (int64_t)cus * 16;
-        const int cand[] = {8, 4, 2, 1};
+        const int c0 = chunk > 0 ? chunk : 16;
         tj = 1;
-        for (int c : cand) {
-            if (sh.v == 2 && c > 4) continue;  // 8 rows of two values per lane would spill
-            const int64_t waves = ceil_div(rows, c) * sh.n_windows * std::max<int64_t>(1, planes / 16);
-            if (waves >= waves_wanted) {
+        for (int c = max_tj; c >= 1; c /= 2) {
+            if (ceil_div(rows, c) * sh.n_windows * ceil_div(planes, c0) >= waves_wanted) {
                 tj = c;
                 break;
             }
         }
     }
+    if (chunk <= 0) {
+        chunk = 16;
+        while (chunk > 4 && ceil_div(rows, tj) * sh.n_windows * ceil_div(planes, chunk) < waves_wanted) chunk /= 2;
+    }
     sh.tj = tj;
     sh.n_jt = (int)ceil_div(rows, tj);
-    int chunk = s->force_chunk;
-    if (chunk <= 0) {
-        const int64_t waves_per_chunkset = (int64_t)sh.n_jt * sh.n_windows;
-        const int64_t waves_wanted = (int64_t)cus * 16;
-        int64_t n_ic = std::max<int64_t>(1, ceil_div(waves_wanted, waves_per_chunkset));
-        chunk = (int)std::max<int64_t>(8, ceil_div(planes, n_ic));
-    }
     sh.chunk = (int)std::min<int64_t>(chunk, planes);
     sh.n_ic = (int)ceil_div(planes, sh.chunk);
     return sh;
 }
 
 template <typename S, typename A, int TJ, int V>
-void launch_sweep_tj(const pic::SweepArgs &a, dim3 grid, dim3 block, hipStream_t st, bool dep) {
-    if (dep)
-        hipLaunchKernelGGL((pic::sweep_kernel<S, A, TJ, true, V>), grid, block, 0, st, a);
-    else
-        hipLaunchKernelGGL((pic::sweep_kernel<S, A, TJ, false, V>), grid, block, 0, st, a);
+void launch_sweep_tj(const pic::SweepArgs &a, dim3 grid, dim3 block, hipStream_t st, bool dep, bool reflect) {
+    if (reflect) {
+        if (dep)
+            hipLaunchKernelGGL((pic::sweep_kernel<S, A, TJ, true, V, true>), grid, block, 0, st, a);
+        else
+            hipLaunchKernelGGL((pic::sweep_kernel<S, A, TJ, false, V, true>), grid, block, 0, st, a);
+    } else {
+        if (dep)
+            hipLaunchKernelGGL((pic::sweep_kernel<S, A, TJ, true, V, false>), grid, block, 0, st, a);
+        else
+            hipLaunchKernelGGL((pic::sweep_kernel<S, A, TJ, false, V, false>), grid, block, 0, st, a);
+    }
 }
 
 template <typename S, typename A, int V>
-int launch_sweep_typed(const pic::SweepArgs &a, int tj, dim3 grid, dim3 block, hipStream_t st, bool dep) {
+int launch_sweep_typed(const pic::SweepArgs &a, int tj, dim3 grid, dim3 block, hipStream_t st, bool dep,
+                       bool reflect) {
     switch (tj) {
-        case 1: launch_sweep_tj<S, A, 1, V>(a, grid, block, st, dep); break;
-        case 2: launch_sweep_tj<S, A, 2, V>(a, grid, block, st, dep); break;
-        case 4: launch_sweep_tj<S, A, 4, V>(a, grid, block, st, dep); break;
+        case 1: launch_sweep_tj<S, A, 1, V>(a, grid, block, st, dep, reflect); break;
+        case 2: launch_sweep_tj<S, A, 2, V>(a, grid, block, st, dep, reflect); break;
+        case 4: launch_sweep_tj<S, A, 4, V>(a, grid, block, st, dep, reflect); break;
         case 8:
-            if constexpr (V == 2) {
-                return fail(PIC_E_INVALID, "rows_per_tile 8 needs one value per lane");
+            if constexpr (V == 2 || sizeof(S) == 8) {
+                return fail(PIC_E_INVALID, "rows_per_tile 8 is for FP32 fields with one value per lane");
             } else {
-                launch_sweep_tj<S, A, 8, V>(a, grid, block, st, dep);
+                launch_sweep_tj<S, A, 8, V>(a, grid, block, st, dep, reflect);
             }
             break;
         default: return fail(PIC_E_INVALID, "unsupported rows_per_tile %d (1, 2, 4 or 8)", tj);
@@ -152,27 +172,43 @@ int launch_sweep_typed(const pic::SweepArgs &a, int tj, dim3 grid, dim3 block, h
     return PIC_OK;
 }
 
+// With timing on, every kernel group is bracketed by a pair of events on the
+// handle's stream; pic_get_timing sums the pairs per kind.
 void record(pic_sim *s, int kind) {
-    if (!s->timing) return;
-    hipEvent_t e;
-    if (hipEventCreate(&e) != hipSuccess) return;
-    hipEventRecord(e, s->stream);
-    s->ev_pool.push_back(e);
-    s->ev_kind.push_back(kind);
+    if (!s->timing || !s->in_run) return;
+    if (s->ev_used == s->ev_pool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return;
+        s->ev_pool.push_back(e);
+        s->ev_kind.push_back(kind);
+    }
+    s->ev_kind[s->ev_used] = kind;
+    if (hipEventRecord(s->ev_pool[s->ev_used], s->stream) == hipSuccess) ++s->ev_used;
 }
 
-// sweep of the current arrays into the next ones (+ deposition if dep)
-int enqueue_sweep(pic_sim *s, bool dep) {
+// sweep of the current arrays into the next ones, adding the deposition
+// density dens (if dep) and folding the step's scalars into out4
+// Sweep step `step`: adds the deposition density dens (if dep), writes its
+// block partials to slot step % 2 and becomes the pending step whose record
+// out4 is folded later; folds the previously pending step at its start.
+int enqueue_sweep(pic_sim *s, int64_t step, bool dep, const double *dens, const double *dep_part, int n_dep,
+                  double *out4) {
     const SweepShape sh = sweep_shape(s);
     const int64_t nblk = sh.blocks();
     if (nblk > s->sweep_part_cap) return fail(PIC_E_INVALID, "sweep grid %lld exceeds partial buffer", (long long)nblk);
+    double *part = s->sweep_part + (step % 2) * s->sweep_part_cap * 3;
     pic::SweepArgs a{};
     a.pos = s->cur_p;
     a.neg = s->cur_n;
     a.pos_out = s->nxt_p;
     a.neg_out = s->nxt_n;
-    a.dens = s->dens;
-    a.partials = s->sweep_part;
+    a.dens = dep ? dens : nullptr;
+    a.partials = part;
+    a.prev_part = s->pending.part;
+    a.prev_n = s->pending.n;
+    a.prev_dep_part = s->pending.dep;
+    a.prev_n_dep = s->pending.n_dep;
+    a.prev_out4 = s->pending.out4;
     a.no_xy = (int)s->p.no_xy;
     a.nzb = (int)s->p.nzb;
     a.nze = (int)s->p.nze;
@@ -195,13 +231,28 @@ int enqueue_sweep(pic_sim *s, bool dep) {
     a.n_cen = s->p.n_cen;
     a.alpha_dt = s->p.alpha_dt;
     const dim3 grid((unsigned)nblk), block((unsigned)(64 * sh.wpb));
+    const bool reflect = s->p.wall == PIC_WALL_REFLECTING;
     record(s, 0);
-    int rc = s->prec == PIC_FP64 ? launch_sweep_typed<double, double, 1>(a, sh.tj, grid, block, s->stream, dep)
-             : sh.v == 2         ? launch_sweep_typed<float, float, 2>(a, sh.tj, grid, block, s->stream, dep)
-                                 : launch_sweep_typed<float, float, 1>(a, sh.tj, grid, block, s->stream, dep);
+    int rc = s->prec == PIC_FP64
+                 ? launch_sweep_typed<double, double, 1>(a, sh.tj, grid, block, s->stream, dep, reflect)
+             : sh.v == 2 ? launch_sweep_typed<float, float, 2>(a, sh.tj, grid, block, s->stream, dep, reflect)
+                         : launch_sweep_typed<float, float, 1>(a, sh.tj, grid, block, s->stream, dep, reflect);
     if (rc) return rc;
     HIP_LAUNCH_CHECK();
     record(s, 0);
+    s->pending = pic_sim::Pending{part, (int)nblk, dep ? dep_part : nullptr, dep ? n_dep : 0, out4};
+    return PIC_OK;
+}
+
+// fold the pending step's partials into its record (end of a run of steps)
+int flush_pending(pic_sim *s) {
+    if (!s->pending.out4) return PIC_OK;
+    record(s, 2);
+    hipLaunchKernelGGL(pic::reduce_kernel, dim3(1), dim3(256), 0, s->stream, s->pending.part, s->pending.n,
+                       s->pending.dep, s->pending.n_dep, s->pending.out4);
+    HIP_LAUNCH_CHECK();
+    record(s, 2);
+    s->pending = pic_sim::Pending{};
     return PIC_OK;
 }
 
@@ -210,11 +261,16 @@ int64_t gather_blocks(const pic_sim *s) {
     return nt * nt;
 }
 
-int enqueue_gather(pic_sim *s, const double *xs, const double *ys, int64_t m) {
+// deposition densities of steps [first, first+count) (count <= batch) into
+// dens[0..count) and dep_part[0..count); offsets are device offsets into xs/ys
+int enqueue_gather(pic_sim *s, const double *xs, const double *ys, const int64_t *offsets, int64_t first,
+                   int64_t count) {
     pic::GatherArgs g{};
     g.xs = xs;
     g.ys = ys;
-    g.m = m;
+    g.offsets = offsets;
+    g.first_step = first;
+    g.n_slots = s->batch + 1;
     g.dens = s->dens;
     g.partials = s->dep_part;
     g.no_xy = (int)s->p.no_xy;
@@ -226,44 +282,16 @@ int enqueue_gather(pic_sim *s, const double *xs, const double *ys, int64_t m) {
     g.mid = s->p.mid_xy;
     g.rsq = s->p.scoring_radius_sq;
     record(s, 1);
-    hipLaunchKernelGGL(pic::gather_kernel, dim3((unsigned)gather_blocks(s)), dim3(256), 0, s->stream, g);
+    hipLaunchKernelGGL(pic::gather_kernel, dim3((unsigned)gather_blocks(s), (unsigned)count), dim3(256), 0, s->stream,
+                       g);
     HIP_LAUNCH_CHECK();
     record(s, 1);
     return PIC_OK;
 }
 
-template <typename S>
-void launch_ring_typed(pic_sim *s, bool dep) {
-    const int64_t n = (4 * s->p.no_xy - 4) * s->p.nzb;
-    const unsigned blocks = (unsigned)std::min<int64_t>(ceil_div(n, 256), 4096);
-    S *np_ = (S *)s->nxt_p, *nn_ = (S *)s->nxt_n;
-    const S *cp_ = (const S *)s->cur_p, *cn_ = (const S *)s->cur_n;
-    const int no_xy = (int)s->p.no_xy, nzb = (int)s->p.nzb, nze = (int)s->p.nze, no_z = (int)s->p.no_z;
-    if (s->p.wall == PIC_WALL_REFLECTING)
-        hipLaunchKernelGGL((pic::ring_kernel<S, true, false>), dim3(blocks), dim3(256), 0, s->stream, np_, nn_, cp_,
-                           cn_, s->dens, no_xy, nzb, nze, no_z);
-    else if (dep)
-        hipLaunchKernelGGL((pic::ring_kernel<S, false, true>), dim3(blocks), dim3(256), 0, s->stream, np_, nn_, cp_,
-                           cn_, s->dens, no_xy, nzb, nze, no_z);
-    else
-        hipLaunchKernelGGL((pic::ring_kernel<S, false, false>), dim3(blocks), dim3(256), 0, s->stream, np_, nn_,
-                           cp_, cn_, s->dens, no_xy, nzb, nze, no_z);
-}
-
-int enqueue_ring(pic_sim *s, bool dep) {
-    if (s->prec == PIC_FP64)
-        launch_ring_typed<double>(s, dep);
-    else
-        launch_ring_typed<float>(s, dep);
-    HIP_LAUNCH_CHECK();
-    return PIC_OK;
-}
-
-int enqueue_reduce(pic_sim *s, bool with_sweep, bool with_dep, double *out4) {
-    const int n_sweep = with_sweep ? (int)sweep_shape(s).blocks() : 0;
-    const int n_dep = with_dep ? (int)gather_blocks(s) : 0;
-    hipLaunchKernelGGL(pic::reduce_kernel, dim3(1), dim3(256), 0, s->stream, s->sweep_part, n_sweep, s->dep_part,
-                       n_dep, out4);
+int enqueue_deposit_reduce(pic_sim *s, double *out4) {
+    hipLaunchKernelGGL(pic::reduce_kernel, dim3(1), dim3(256), 0, s->stream, s->sweep_part, 0, s->dep_part,
+                       (int)gather_blocks(s), out4);  // slot 0: step 0 of a one-step gather
     HIP_LAUNCH_CHECK();
     return PIC_OK;
 }
@@ -282,6 +310,12 @@ int check_params(const pic_params *p, int precision) {
     if (p->wall != PIC_WALL_ABSORBING && p->wall != PIC_WALL_REFLECTING) return fail(PIC_E_INVALID, "wall must be PIC_WALL_ABSORBING or PIC_WALL_REFLECTING");
     if (!(p->cutoff_voxels > 0.0)) return fail(PIC_E_INVALID, "cutoff_voxels must be positive");
     return PIC_OK;
+}
+
+// steps per gather launch: as many as fit in ~256 MB of density, at most 128
+int64_t batch_steps(const pic_params *p) {
+    const int64_t per = p->no_xy * p->no_xy * 8;
+    return std::max<int64_t>(1, std::min<int64_t>(128, (int64_t)(256LL << 20) / per));
 }
 
 size_t partial_cap_bytes(const pic_params *p) {
@@ -325,7 +359,8 @@ int pic_mem_required(const pic_params *p, int precision, size_t *bytes) {
     const size_t vox = (size_t)p->no_xy * p->no_xy * p->nzb;
     const size_t xy = (size_t)p->no_xy * p->no_xy;
     const size_t gt = (size_t)ceil_div(p->no_xy, pic::kGT);
-    *bytes = 4 * vox * elem + xy * 8 + partial_cap_bytes(p) + gt * gt * 8 + 64;
+    const size_t b = (size_t)batch_steps(p) + 1;
+    *bytes = 4 * vox * elem + b * (xy * 8 + gt * gt * 8) + 2 * partial_cap_bytes(p) + 256;
     return PIC_OK;
 }
 
@@ -363,12 +398,16 @@ int pic_create(const pic_params *p, int precision, pic_sim **out) {
     }
     const size_t xy = (size_t)p->no_xy * p->no_xy;
     s->sweep_part_cap = (int64_t)(partial_cap_bytes(p) / (3 * sizeof(double)));
-    s->dep_part_cap = gather_blocks(s);
-    if (hipMalloc(&s->dens, xy * 8) != hipSuccess || hipMalloc(&s->sweep_part, s->sweep_part_cap * 3 * 8) != hipSuccess ||
-        hipMalloc(&s->dep_part, s->dep_part_cap * 8) != hipSuccess || hipMalloc(&s->scratch4, 8 * 8) != hipSuccess)
+    s->batch = batch_steps(p);
+    const int64_t slots = s->batch + 1;
+    if (hipMalloc(&s->dens, slots * xy * 8) != hipSuccess ||
+        hipMalloc(&s->sweep_part, 2 * s->sweep_part_cap * 3 * 8) != hipSuccess ||
+        hipMalloc(&s->dep_part, slots * gather_blocks(s) * 8) != hipSuccess ||
+        hipMalloc(&s->scratch4, 8 * 8) != hipSuccess)
         return cleanup(fail(PIC_E_NOMEM, "hipMalloc of scratch failed"));
-    hipMemsetAsync(s->dens, 0, xy * 8, s->stream);
-    hipMemsetAsync(s->scratch4, 0, 64, s->stream);
+    if (hipMemsetAsync(s->dens, 0, slots * xy * 8, s->stream) != hipSuccess ||
+        hipMemsetAsync(s->scratch4, 0, 64, s->stream) != hipSuccess)
+        return cleanup(fail(PIC_E_HIP, "hipMemsetAsync failed"));
     if (hipStreamSynchronize(s->stream) != hipSuccess) return cleanup(fail(PIC_E_HIP, "initialisation failed"));
     *out = s;
     return PIC_OK;
@@ -378,7 +417,7 @@ int pic_destroy(pic_sim *s) {
     if (!s) return PIC_OK;
     if (s->stream) hipStreamSynchronize(s->stream);
     void *bufs[] = {s->cur_p, s->cur_n, s->nxt_p, s->nxt_n, s->dens, s->sweep_part, s->dep_part,
-                    s->scratch4, s->per_step, s->xs, s->ys};
+                    s->scratch4, s->per_step, s->xs, s->ys, s->offsets_d};
     for (void *b : bufs)
         if (b) hipFree(b);
     for (hipEvent_t e : s->ev_pool) hipEventDestroy(e);
@@ -396,14 +435,20 @@ int pic_upload_tracks(pic_sim *s, const double *xs, const double *ys, int64_t n_
     for (int64_t i = 0; i < n_steps; ++i)
         if (offsets[i + 1] < offsets[i]) return fail(PIC_E_INVALID, "offsets must be non-decreasing");
     HIP_TRY(hipStreamSynchronize(s->stream));
+    s->pending = pic_sim::Pending{};
     if (s->xs) hipFree(s->xs);
     if (s->ys) hipFree(s->ys);
     if (s->per_step) hipFree(s->per_step);
+    if (s->offsets_d) hipFree(s->offsets_d);
     s->xs = s->ys = s->per_step = nullptr;
+    s->offsets_d = nullptr;
+    s->n_steps = 0;
     const size_t tb = (size_t)std::max<int64_t>(n_tracks, 1) * 8;
     if (hipMalloc(&s->xs, tb) != hipSuccess || hipMalloc(&s->ys, tb) != hipSuccess ||
-        hipMalloc(&s->per_step, (size_t)std::max<int64_t>(n_steps, 1) * 4 * 8) != hipSuccess)
+        hipMalloc(&s->per_step, (size_t)std::max<int64_t>(n_steps, 1) * 4 * 8) != hipSuccess ||
+        hipMalloc(&s->offsets_d, (size_t)(n_steps + 1) * 8) != hipSuccess)
         return fail(PIC_E_NOMEM, "hipMalloc for %lld tracks failed", (long long)n_tracks);
+    HIP_TRY(hipMemcpyAsync(s->offsets_d, offsets, (size_t)(n_steps + 1) * 8, hipMemcpyHostToDevice, s->stream));
     if (n_tracks > 0) {
         HIP_TRY(hipMemcpyAsync(s->xs, xs, n_tracks * 8, hipMemcpyHostToDevice, s->stream));
         HIP_TRY(hipMemcpyAsync(s->ys, ys, n_tracks * 8, hipMemcpyHostToDevice, s->stream));
@@ -421,22 +466,27 @@ int pic_run_steps(pic_sim *s, int64_t first, int64_t count) {
     if (first < 0 || count < 0 || first + count > s->n_steps)
         return fail(PIC_E_INVALID, "steps [%lld, %lld) outside the uploaded %lld", (long long)first,
                     (long long)(first + count), (long long)s->n_steps);
-    if (s->timing) {
-        for (hipEvent_t e : s->ev_pool) hipEventDestroy(e);
-        s->ev_pool.clear();
-        s->ev_kind.clear();
+    HIP_TRY(hipStreamSynchronize(s->stream));  // events of an earlier call may still be pending
+    s->ev_used = 0;
+    s->in_run = true;
+    int rc = PIC_OK;
+    const int n_tiles = (int)gather_blocks(s);
+    const int64_t xy = s->p.no_xy * s->p.no_xy;
+    const int64_t slots = s->batch + 1;
+    for (int64_t b0 = first; b0 < first + count && rc == PIC_OK; b0 += s->batch) {
+        const int64_t b1 = std::min(first + count, b0 + s->batch);
+        if (s->offsets[b1] > s->offsets[b0]) rc = enqueue_gather(s, s->xs, s->ys, s->offsets_d, b0, b1 - b0);
+        for (int64_t step = b0; step < b1 && rc == PIC_OK; ++step) {
+            const bool dep = s->offsets[step + 1] > s->offsets[step];
+            const int64_t slot = step % slots;
+            rc = enqueue_sweep(s, step, dep, s->dens + slot * xy, s->dep_part + slot * n_tiles, n_tiles,
+                               s->per_step + 4 * step);
+            if (!rc) swap_buffers(s);
+        }
     }
-    for (int64_t step = first; step < first + count; ++step) {
-        const int64_t m = s->offsets[step + 1] - s->offsets[step];
-        const bool dep = m > 0;
-        int rc;
-        if (dep && (rc = enqueue_gather(s, s->xs + s->offsets[step], s->ys + s->offsets[step], m))) return rc;
-        if ((rc = enqueue_sweep(s, dep))) return rc;
-        if ((rc = enqueue_ring(s, dep))) return rc;
-        if ((rc = enqueue_reduce(s, true, dep, s->per_step + 4 * step))) return rc;
-        swap_buffers(s);
-    }
-    return PIC_OK;
+    if (!rc) rc = flush_pending(s);
+    s->in_run = false;
+    return rc;
 }
 
 int pic_read_per_step(pic_sim *s, int64_t first, int64_t count, double *out) {
@@ -454,15 +504,20 @@ int pic_deposit(pic_sim *s, const double *xs, const double *ys, int64_t m, doubl
     double host4[4] = {0, 0, 0, 0};
     if (m > 0) {
         double *dx = nullptr, *dy = nullptr;
-        if (hipMalloc(&dx, m * 8) != hipSuccess || hipMalloc(&dy, m * 8) != hipSuccess) {
+        int64_t *doff = nullptr;
+        const int64_t off[2] = {0, m};
+        if (hipMalloc(&dx, m * 8) != hipSuccess || hipMalloc(&dy, m * 8) != hipSuccess ||
+            hipMalloc(&doff, sizeof off) != hipSuccess) {
             if (dx) hipFree(dx);
+            if (dy) hipFree(dy);
             return fail(PIC_E_NOMEM, "hipMalloc for %lld tracks failed", (long long)m);
         }
         int rc = PIC_OK;
         if (hipMemcpyAsync(dx, xs, m * 8, hipMemcpyHostToDevice, s->stream) != hipSuccess ||
-            hipMemcpyAsync(dy, ys, m * 8, hipMemcpyHostToDevice, s->stream) != hipSuccess)
+            hipMemcpyAsync(dy, ys, m * 8, hipMemcpyHostToDevice, s->stream) != hipSuccess ||
+            hipMemcpyAsync(doff, off, sizeof off, hipMemcpyHostToDevice, s->stream) != hipSuccess)
             rc = fail(PIC_E_HIP, "track upload failed");
-        if (!rc) rc = enqueue_gather(s, dx, dy, m);
+        if (!rc) rc = enqueue_gather(s, dx, dy, doff, 0, 1);
         if (!rc) {
             const int64_t n = s->p.no_xy * s->p.no_xy * s->p.no_z;
             const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, 256), 8192));
@@ -475,13 +530,14 @@ int pic_deposit(pic_sim *s, const double *xs, const double *ys, int64_t m, doubl
                                    (float *)s->cur_p, (float *)s->cur_n, s->dens, no_xy, nzb, nze, no_z);
             if (hipGetLastError() != hipSuccess) rc = fail(PIC_E_HIP, "broadcast launch failed");
         }
-        if (!rc) rc = enqueue_reduce(s, false, true, s->scratch4);
+        if (!rc) rc = enqueue_deposit_reduce(s, s->scratch4);
         if (!rc && (hipMemcpyAsync(host4, s->scratch4, 32, hipMemcpyDeviceToHost, s->stream) != hipSuccess ||
                     hipStreamSynchronize(s->stream) != hipSuccess))
             rc = fail(PIC_E_HIP, "deposit readback failed");
         hipStreamSynchronize(s->stream);
         hipFree(dx);
         hipFree(dy);
+        hipFree(doff);
         if (rc) return rc;
     }
     if (inserted) *inserted = host4[0];
@@ -491,9 +547,9 @@ int pic_deposit(pic_sim *s, const double *xs, const double *ys, int64_t m, doubl
 int pic_step(pic_sim *s, double out3[3]) {
     if (!s) return fail(PIC_E_INVALID, "sim is NULL");
     int rc;
-    if ((rc = enqueue_sweep(s, false))) return rc;
-    if ((rc = enqueue_ring(s, false))) return rc;
-    if ((rc = enqueue_reduce(s, true, false, s->scratch4))) return rc;
+    if ((rc = flush_pending(s))) return rc;
+    if ((rc = enqueue_sweep(s, 0, false, nullptr, nullptr, 0, s->scratch4))) return rc;
+    if ((rc = flush_pending(s))) return rc;
     swap_buffers(s);
     double host4[4];
     HIP_TRY(hipMemcpyAsync(host4, s->scratch4, 32, hipMemcpyDeviceToHost, s->stream));
@@ -508,6 +564,18 @@ int pic_step(pic_sim *s, double out3[3]) {
 
 int pic_read_fields(pic_sim *s, double *pos, double *neg) {
     if (!s) return fail(PIC_E_INVALID, "sim is NULL");
+    if (s->p.wall == PIC_WALL_REFLECTING) {
+        // the sweep reads the reflecting ring through the interior; store it now
+        const int64_t n = (4 * s->p.no_xy - 4) * s->p.nzb;
+        const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, 256), 4096));
+        if (s->prec == PIC_FP64)
+            hipLaunchKernelGGL(pic::ring_mirror_kernel<double>, dim3(blocks), dim3(256), 0, s->stream,
+                               (double *)s->cur_p, (double *)s->cur_n, (int)s->p.no_xy, (int)s->p.nzb);
+        else
+            hipLaunchKernelGGL(pic::ring_mirror_kernel<float>, dim3(blocks), dim3(256), 0, s->stream,
+                               (float *)s->cur_p, (float *)s->cur_n, (int)s->p.no_xy, (int)s->p.nzb);
+        HIP_LAUNCH_CHECK();
+    }
     HIP_TRY(hipStreamSynchronize(s->stream));
     void *src[2] = {s->cur_p, s->cur_n};
     double *dst[2] = {pos, neg};
@@ -560,6 +628,8 @@ int pic_set_tiling(pic_sim *s, int rows_per_tile, int planes_per_chunk, int valu
     s->force_v1 = values_per_lane == 1;
     if (rows_per_tile && rows_per_tile != 1 && rows_per_tile != 2 && rows_per_tile != 4 && rows_per_tile != 8)
         return fail(PIC_E_INVALID, "rows_per_tile must be 0, 1, 2, 4 or 8");
+    if (rows_per_tile == 8 && (s->prec != PIC_FP32 || values_per_lane != 1))
+        return fail(PIC_E_INVALID, "rows_per_tile 8 needs FP32 fields and values_per_lane 1");
     s->force_tj = rows_per_tile;
     s->force_chunk = planes_per_chunk;
     const SweepShape sh = sweep_shape(s);
@@ -580,24 +650,42 @@ int pic_get_timing(pic_sim *s, double *sweep_ms, int64_t *sweep_launches, double
                    double *other_ms) {
     if (!s) return fail(PIC_E_INVALID, "sim is NULL");
     HIP_TRY(hipStreamSynchronize(s->stream));
-    double ts = 0, td = 0;
-    int64_t ns = 0, nd = 0;
-    for (size_t q = 0; q + 1 < s->ev_pool.size(); q += 2) {
+    double t[3] = {0, 0, 0};
+    int64_t n[3] = {0, 0, 0};
+    for (size_t q = 0; q + 1 < s->ev_used; q += 2) {
         float ms = 0;
         HIP_TRY(hipEventElapsedTime(&ms, s->ev_pool[q], s->ev_pool[q + 1]));
-        if (s->ev_kind[q] == 0) {
-            ts += ms;
-            ++ns;
-        } else {
-            td += ms;
-            ++nd;
-        }
+        const int kind = s->ev_kind[q];
+        t[kind] += ms;
+        ++n[kind];
     }
-    if (sweep_ms) *sweep_ms = ts;
-    if (sweep_launches) *sweep_launches = ns;
-    if (deposit_ms) *deposit_ms = td;
-    if (deposit_launches) *deposit_launches = nd;
-    if (other_ms) *other_ms = 0.0;
+    if (sweep_ms) *sweep_ms = t[0];
+    if (sweep_launches) *sweep_launches = n[0];
+    if (deposit_ms) *deposit_ms = t[1];
+    if (deposit_launches) *deposit_launches = n[1];
+    if (other_ms) *other_ms = t[2];
+    return PIC_OK;
+}
+
+int pic_reset(pic_sim *s) {
+    if (!s) return fail(PIC_E_INVALID, "sim is NULL");
+    s->pending = pic_sim::Pending{};
+    void *bufs[4] = {s->cur_p, s->cur_n, s->nxt_p, s->nxt_n};
+    const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(s->n_vox, 256), 16384));
+    for (void *b : bufs) {
+        if (s->prec == PIC_FP64)
+            hipLaunchKernelGGL(pic::fill_kernel<double>, dim3(blocks), dim3(256), 0, s->stream, (double *)b, s->n_vox);
+        else
+            hipLaunchKernelGGL(pic::fill_kernel<float>, dim3(blocks), dim3(256), 0, s->stream, (float *)b, s->n_vox);
+        HIP_LAUNCH_CHECK();
+    }
+    if (s->per_step) {
+        const int64_t n = std::max<int64_t>(s->n_steps, 1) * 4;
+        hipLaunchKernelGGL(pic::fill_kernel<double>, dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, 256), 4096))),
+                           dim3(256), 0, s->stream, s->per_step, n);
+        HIP_LAUNCH_CHECK();
+    }
+    HIP_TRY(hipStreamSynchronize(s->stream));
     return PIC_OK;
 }
 

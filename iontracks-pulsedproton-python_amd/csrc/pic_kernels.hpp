@@ -6,6 +6,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace pic {
 
 constexpr int kWave = 64;
@@ -54,13 +56,77 @@ __device__ __forceinline__ void block_sum3(double &a, double &b, double &c) {
     }
 }
 
+// Neumaier-compensated accumulation of one term.
+struct Comp {
+    double s = 0.0, c = 0.0;
+    __device__ __forceinline__ void add(double x) {
+        const double t = s + x;
+        c += fabs(s) >= fabs(x) ? (s - t) + x : (x - t) + s;
+        s = t;
+    }
+    __device__ __forceinline__ void add(const Comp &o) {
+        add(o.s);
+        c += o.c;
+    }
+    __device__ __forceinline__ double value() const { return s + c; }
+};
+
+// Fold (s, c) pairs across the wave by shuffles, fixed tree order.
+__device__ __forceinline__ Comp wave_comp(Comp x) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        Comp y{__shfl_xor(x.s, off, kWave), __shfl_xor(x.c, off, kWave)};
+        x.add(y);
+    }
+    return x;
+}
+
+// Sum a [n3][3] table of block partials and, if given, an [n_dep] table,
+// over the whole block in a fixed order with compensated
+// summation; thread 0 receives the four totals. Used by the last block of a
+// sweep to turn block partials into the step's scalars without another launch.
+__device__ void block_final4(const double *part3, int n3, const double *dep, int n_dep, double out[4]) {
+    __shared__ double s_s[4][16], s_c[4][16];
+    Comp q[4];
+    for (int b = threadIdx.x; b < n_dep; b += blockDim.x) q[0].add(dep[b]);
+    for (int b = threadIdx.x; b < n3; b += blockDim.x) {
+        q[1].add(part3[3 * (int64_t)b + 0]);
+        q[2].add(part3[3 * (int64_t)b + 1]);
+        q[3].add(part3[3 * (int64_t)b + 2]);
+    }
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        q[v] = wave_comp(q[v]);
+        if (lane == 0) {
+            s_s[v][wave] = q[v].s;
+            s_c[v][wave] = q[v].c;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nw = (blockDim.x + kWave - 1) / kWave;
+        for (int v = 0; v < 4; ++v) {
+            Comp t{s_s[v][0], s_c[v][0]};
+            for (int w = 1; w < nw; ++w) t.add(Comp{s_s[v][w], s_c[v][w]});
+            out[v] = t.value();
+        }
+    }
+}
+
 struct SweepArgs {
     const void *pos;
     const void *neg;
     void *pos_out;
     void *neg_out;
-    const double *dens;  // gf-scaled 2D density of this step, or null
-    double *partials;    // [gridDim.x][3]: recombined, total+, total-
+    const double *dens;      // gf-scaled 2D density of this step, or null
+    double *partials;        // [gridDim.x][3]: recombined, total+, total-
+    // the previous step's partials, folded into its record by block 0
+    const double *prev_part;      // [prev_n][3], or null
+    int prev_n;
+    const double *prev_dep_part;  // [prev_n_dep], or null
+    int prev_n_dep;
+    double *prev_out4;            // (injected, recombined, total+, total-), or null
     int no_xy, nzb, nze, no_z;
     int n_windows;  // k windows of kOut outputs covering k = 1..nzb-2
     int wpb;        // windows (waves) per block
@@ -75,7 +141,8 @@ struct SweepArgs {
 typedef float f32x2 __attribute__((ext_vector_type(2)));
 
 // One Lax-Wendroff step of both species, fused with the gap-layer deposition
-// of this step (DEP) and the scored reduction.
+// of this step (DEP), the lateral-ring update of the next arrays and the
+// scored reduction.
 //
 // Work decomposition: a wave owns a k window, TJ consecutive interior rows j of
 // one tile, and marches through a chunk of i planes. Each lane holds V
@@ -92,12 +159,35 @@ typedef float f32x2 __attribute__((ext_vector_type(2)));
 // k neighbours come from the adjacent lanes by wave shuffles. No LDS staging
 // and no barriers in the march: waves are independent.
 //
+// Lateral ring (the sweep proper writes only the interior):
+//  - REFLECT: the ring is never stored. A zero-gradient wall makes every ring
+//    voxel equal the nearest interior column -- i planes mirrored first, then
+//    j planes, a corner taking the diagonal column (state.py:113-154) -- so a
+//    load of ring voxel (i', j') reads interior column (clamp(i'), clamp(j'))
+//    and adds the deposition at (i', j') itself, which the reference deposits
+//    into the ring after mirroring. The stored ring is filled in only when the
+//    fields are read back (ring_mirror_kernel).
+//  - absorbing: the tiles and chunks at the grid edge carry the ring values
+//    they load -- current value plus this step's deposition, which the
+//    reference also adds to the ring -- into the next arrays
+//    (state.py:157-176, solver_numba_parallel.py:691-695).
+// The k end planes are zero in both buffers and are never written.
+//
+// Reduction: each block writes its three scored sums to its slot; block 0 of
+// the NEXT step's launch folds them (with that step's deposition partials) in
+// a fixed order with compensated FP64 summation into the step's record, so
+// the kernel boundary orders the partials and no device-wide fence is needed.
+// The last step of a run is folded by reduce_kernel.
+//
 // The arithmetic is the reference's, term for term and in the same order
 // (solver_numba_parallel.py:374-405); with S = A = double and contraction off
-// a voxel update is bit-identical to the reference's FP64 sweep. With
-// S = A = float the stencil runs in FP32 while the recombination product, the
-// deposited density and every scored sum stay FP64.
-template <typename S, typename A, int TJ, bool DEP, int V>
+// a voxel update is bit-identical to the reference's FP64 sweep. With FP32
+// fields (S = float) the stencil runs in FP32 in flux form (see the update):
+// the reference's weighted sum with FP32-rounded weights would not conserve
+// charge -- the weights no longer sum to one -- and that per-step drift, not
+// the FP32 rounding of each value, is what would move k_s. The recombination
+// product and every scored sum stay FP64.
+template <typename S, typename A, int TJ, bool DEP, int V, bool REFLECT>
 __global__ void __launch_bounds__(512) sweep_kernel(SweepArgs a) {
     static_assert(V == 1 || (V == 2 && sizeof(S) == 4), "two values per lane is the FP32 layout");
     constexpr int WIN = kWave * V - 2;  // outputs per window
@@ -106,94 +196,183 @@ __global__ void __launch_bounds__(512) sweep_kernel(SweepArgs a) {
     const int rest = logical / a.n_kg;
     const int jt = rest % a.n_jt;
     const int ic = rest / a.n_jt;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // the wave index is uniform across the wave; saying so lets the compiler
+    // keep the whole march in scalar control flow and scalar loads
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int win = kg * a.wpb + wave;
+
+    if (blockIdx.x == 0 && a.prev_out4) {
+        double out[4];
+        block_final4(a.prev_part, a.prev_n, a.prev_dep_part, a.prev_n_dep, out);
+        if (threadIdx.x == 0)
+            for (int v = 0; v < 4; ++v) a.prev_out4[v] = out[v];
+    }
 
     double acc_r = 0.0, acc_p = 0.0, acc_n = 0.0;
     if (win < a.n_windows) {
         const int no_xy = a.no_xy, nzb = a.nzb;
         const int k0 = win * WIN + V * lane;
         const bool k_in = k0 < nzb;  // with V = 2, nzb is even so k0+1 < nzb too
-        bool k_out[V], k_gap[V];
+        bool k_out[V], k_gap[V], k_score[V];
 #pragma unroll
         for (int v = 0; v < V; ++v) {
             const int q = V * lane + v, k = k0 + v;
             k_out[v] = q >= 1 && q <= kWave * V - 2 && k >= 1 && k <= nzb - 2;
             k_gap[v] = k >= a.nze && k < a.nze + a.no_z;
+            k_score[v] = k_out[v] && k_gap[v];
         }
         const int j0 = 1 + jt * TJ;
         const int nrows = min(TJ, no_xy - 1 - j0);  // valid own rows
         const int i0 = 1 + ic * a.chunk;
         const int i1 = min(i0 + a.chunk, no_xy - 1);
-        const S *P = static_cast<const S *>(a.pos);
-        const S *N = static_cast<const S *>(a.neg);
-        S *PO = static_cast<S *>(a.pos_out);
-        S *NO = static_cast<S *>(a.neg_out);
+        const bool first_tile = j0 == 1, last_tile = j0 + nrows == no_xy - 1;
+        // four distinct arrays: restrict lets the wave-uniform density reads
+        // go through the scalar cache
+        const S *__restrict__ P = static_cast<const S *>(a.pos);
+        const S *__restrict__ N = static_cast<const S *>(a.neg);
+        S *__restrict__ PO = static_cast<S *>(a.pos_out);
+        S *__restrict__ NO = static_cast<S *>(a.neg_out);
+        const double *__restrict__ dens = a.dens;
         const int64_t plane = (int64_t)no_xy * nzb;
+        const float f_pzm = (float)a.p_zm, f_pzp = (float)a.p_zp, f_plat = (float)a.p_lat;
+        const float f_nzm = (float)a.n_zm, f_nzp = (float)a.n_zp, f_nlat = (float)a.n_lat;
 
-        // values of (i, j, k0..k0+V-1) with this step's deposition added in the
-        // gap layers; the row base is wave-uniform, so each load is one scalar
-        // base plus the lane's k offset
-        auto load2 = [&](int i, int j, A (&vp)[V], A (&vn)[V]) {
-            const int64_t row = (int64_t)i * plane + (int64_t)j * nzb;
-            S sp[V], sn[V];
+        // raw values of (i, j, k0..k0+V-1); the row base is wave-uniform, so
+        // each load is one scalar base plus the lane's k offset. Under a
+        // reflecting wall a ring row reads its mirror column.
+        auto load_raw = [&](int i, int j, S (&vp)[V], S (&vn)[V]) {
+            const int si = REFLECT ? min(max(i, 1), no_xy - 2) : i;
+            const int sj = REFLECT ? min(max(j, 1), no_xy - 2) : j;
+            const int64_t row = (int64_t)si * plane + (int64_t)sj * nzb;
 #pragma unroll
-            for (int v = 0; v < V; ++v) sp[v] = sn[v] = (S)0;
+            for (int v = 0; v < V; ++v) vp[v] = vn[v] = (S)0;
             if (k_in) {
                 if constexpr (V == 2) {
                     const f32x2 tp = *reinterpret_cast<const f32x2 *>(P + row + k0);
                     const f32x2 tn = *reinterpret_cast<const f32x2 *>(N + row + k0);
-                    sp[0] = tp.x;
-                    sp[1] = tp.y;
-                    sn[0] = tn.x;
-                    sn[1] = tn.y;
+                    vp[0] = tp.x;
+                    vp[1] = tp.y;
+                    vn[0] = tn.x;
+                    vn[1] = tn.y;
                 } else {
-                    sp[0] = (P + row)[k0];
-                    sn[0] = (N + row)[k0];
-                }
-            }
-            double d = 0.0;
-            if (DEP) d = a.dens[(int64_t)i * no_xy + j];
-#pragma unroll
-            for (int v = 0; v < V; ++v) {
-                if (DEP && k_gap[v]) {
-                    vp[v] = (A)((double)sp[v] + d);
-                    vn[v] = (A)((double)sn[v] + d);
-                } else {
-                    vp[v] = (A)sp[v];
-                    vn[v] = (A)sn[v];
+                    vp[0] = (P + row)[k0];
+                    vn[0] = (N + row)[k0];
                 }
             }
         };
-        auto load_rows = [&](int i, A (&rp)[TJ + 2][V], A (&rn)[TJ + 2][V]) {
+        // This step's deposition of the TJ+2 rows of a plane: one vector load in
+        // which lane r holds row j0-1+r (the rows are contiguous in the 2D
+        // density), read back per row with readlane into scalar registers.
+        static_assert(TJ + 2 <= kWave, "one lane per row");
+        auto load_dens = [&](int i) -> double {
+            if (!DEP) return 0.0;
+            const int r = min(lane, nrows + 1);
+            return dens[(int64_t)i * no_xy + (j0 - 1 + r)];
+        };
+        auto row_dens = [&](double dv_lanes, int r) -> double {
+            const int lo = __builtin_amdgcn_readlane(__double2loint(dv_lanes), r);
+            const int hi = __builtin_amdgcn_readlane(__double2hiint(dv_lanes), r);
+            return __hiloint2double(hi, lo);
+        };
+        // branch-free: x + 0 == x exactly, so the non-gap layers are unchanged
+        auto add_dep = [&](double d, S (&vp)[V], S (&vn)[V]) {
+            if (DEP) {
+#pragma unroll
+                for (int v = 0; v < V; ++v) {
+                    if constexpr (sizeof(S) == 8) {
+                        const double dv = k_gap[v] ? d : 0.0;
+                        vp[v] = vp[v] + dv;
+                        vn[v] = vn[v] + dv;
+                    } else {
+                        // FP32 fields: the density is rounded once, then added in FP32
+                        const float dv = k_gap[v] ? (float)d : 0.0f;
+                        vp[v] = vp[v] + dv;
+                        vn[v] = vn[v] + dv;
+                    }
+                }
+            }
+        };
+        // rows j0-1 .. j0+nrows of plane i; the deposition is added later, when
+        // the plane is consumed, so the loads stay in flight meanwhile
+        auto load_rows = [&](int i, S (&rp)[TJ + 2][V], S (&rn)[TJ + 2][V], double &rd) {
 #pragma unroll
             for (int r = 0; r < TJ + 2; ++r) {
 #pragma unroll
-                for (int v = 0; v < V; ++v) rp[r][v] = rn[r][v] = (A)0;
-                if (r <= nrows + 1) load2(i, j0 + r - 1, rp[r], rn[r]);
+                for (int v = 0; v < V; ++v) rp[r][v] = rn[r][v] = (S)0;
+                if (r <= nrows + 1) load_raw(i, j0 + r - 1, rp[r], rn[r]);
+            }
+            rd = load_dens(i);
+        };
+        auto add_dep_rows = [&](S (&rp)[TJ + 2][V], S (&rn)[TJ + 2][V], double rd) {
+            if (DEP) {
+#pragma unroll
+                for (int r = 0; r < TJ + 2; ++r)
+                    if (r <= nrows + 1) add_dep(row_dens(rd, r), rp[r], rn[r]);
+            }
+        };
+        // store V values of (i, j) to the next arrays, lanes/values that own k
+        auto store2 = [&](int i, int j, const S (&vp)[V], const S (&vn)[V]) {
+            const int64_t row = (int64_t)i * plane + (int64_t)j * nzb;
+            if constexpr (V == 2) {
+                if (k_out[0] && k_out[1]) {
+                    *reinterpret_cast<f32x2 *>(PO + row + k0) = f32x2{vp[0], vp[1]};
+                    *reinterpret_cast<f32x2 *>(NO + row + k0) = f32x2{vn[0], vn[1]};
+                    return;
+                }
+            }
+#pragma unroll
+            for (int v = 0; v < V; ++v)
+                if (k_out[v]) {
+                    (PO + row)[k0 + v] = vp[v];
+                    (NO + row)[k0 + v] = vn[v];
+                }
+        };
+        // absorbing wall: carry rows r of a loaded plane that lie on the ring
+        auto carry_ring_rows = [&](int i, const S (&rp)[TJ + 2][V], const S (&rn)[TJ + 2][V], bool whole_plane) {
+#pragma unroll
+            for (int r = 0; r < TJ + 2; ++r) {
+                const bool own = r >= 1 && r <= nrows;
+                const bool ring_row = (r == 0 && first_tile) || (r == nrows + 1 && last_tile);
+                if ((whole_plane && own) || ring_row) store2(i, j0 + r - 1, rp[r], rn[r]);
             }
         };
 
-        A pp[TJ][V], pn[TJ][V];          // plane i-1, own rows
-        A cp[TJ + 2][V], cn[TJ + 2][V];  // plane i, rows j0-1 .. j0+TJ
-        A xp[TJ + 2][V], xn[TJ + 2][V];  // plane i+1
-        A fp[TJ + 2][V], fn[TJ + 2][V];  // plane i+2, in flight
-#pragma unroll
-        for (int r = 0; r < TJ; ++r) {
-#pragma unroll
-            for (int v = 0; v < V; ++v) pp[r][v] = pn[r][v] = (A)0;
-            if (r < nrows) load2(i0 - 1, j0 + r, pp[r], pn[r]);
-        }
-        load_rows(i0, cp, cn);
-        load_rows(i0 + 1, xp, xn);
+        // Four plane buffers in rotation: at plane i, buffer (i - i0) % 4 holds
+        // plane i-1, the next two hold planes i and i+1, and the fourth is
+        // being loaded with plane i+2. The march is unrolled by four so the
+        // rotation is a renaming of registers, not a copy.
+        S bp[4][TJ + 2][V], bn[4][TJ + 2][V];
+        double fd;  // deposition of the rows in flight, lane r = row r
+        load_rows(i0 - 1, bp[0], bn[0], fd);
+        add_dep_rows(bp[0], bn[0], fd);
+        if (!REFLECT && i0 == 1) carry_ring_rows(0, bp[0], bn[0], true);
+        load_rows(i0, bp[1], bn[1], fd);
+        add_dep_rows(bp[1], bn[1], fd);
+        load_rows(i0 + 1, bp[2], bn[2], fd);
+        add_dep_rows(bp[2], bn[2], fd);
 
-        for (int i = i0; i < i1; ++i) {
-            if (i + 1 < i1) load_rows(i + 2, fp, fn);
+        // per-lane sums of the scored voxels; lanes whose k is never scored are
+        // masked once at the end (k_score is fixed for the whole march)
+        double sr[V], sp[V], sn[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) sr[v] = sp[v] = sn[v] = 0.0;
+
+        auto plane_step = [&](int i, auto Pc, auto Cc, auto Xc, auto Fc) {
+            constexpr int PB = decltype(Pc)::value, CB = decltype(Cc)::value;
+            constexpr int XB = decltype(Xc)::value, FB = decltype(Fc)::value;
+            const bool more = i + 1 < i1;
+            if (more) load_rows(i + 2, bp[FB], bn[FB], fd);
+            if (!REFLECT) {
+                carry_ring_rows(i, bp[CB], bn[CB], false);
+                if (i == no_xy - 2) carry_ring_rows(no_xy - 1, bp[XB], bn[XB], true);
+            }
             const double di = (double)i - a.mid;
             const double di2 = di * di;
 #pragma unroll
             for (int r = 0; r < TJ; ++r) {
-                A p_km[V], p_kp[V], n_km[V], n_kp[V];
+                const S(&cp)[TJ + 2][V] = bp[CB];
+                const S(&cn)[TJ + 2][V] = bn[CB];
+                S p_km[V], p_kp[V], n_km[V], n_kp[V];
                 if constexpr (V == 2) {
                     p_km[0] = __shfl_up(cp[r + 1][1], 1, kWave);
                     p_kp[0] = cp[r + 1][1];
@@ -212,63 +391,72 @@ __global__ void __launch_bounds__(512) sweep_kernel(SweepArgs a) {
                 if (r < nrows) {
                     const int j = j0 + r;
                     const double dj = (double)j - a.mid;
-                    const bool inside = di2 + dj * dj < a.rsq;
+                    const bool inside = di2 + dj * dj < a.rsq;  // wave-uniform
                     S po[V], no[V];
+                    double rec[V];
 #pragma unroll
                     for (int v = 0; v < V; ++v) {
-                        const A p = cp[r + 1][v], n = cn[r + 1][v];
-                        const A p_new = (A)a.p_zm * p_km[v] + (A)a.p_zp * p_kp[v] + (A)a.p_lat * cp[r][v] +
-                                        (A)a.p_lat * cp[r + 2][v] + (A)a.p_lat * pp[r][v] +
-                                        (A)a.p_lat * xp[r + 1][v] + (A)a.p_cen * p;
-                        const A n_new = (A)a.n_zm * n_kp[v] + (A)a.n_zp * n_km[v] + (A)a.n_lat * cn[r][v] +
-                                        (A)a.n_lat * cn[r + 2][v] + (A)a.n_lat * pn[r][v] +
-                                        (A)a.n_lat * xn[r + 1][v] + (A)a.n_cen * n;
-                        const double rec = a.alpha_dt * (double)p * (double)n;
-                        po[v] = (S)((double)p_new - rec);
-                        no[v] = (S)((double)n_new - rec);
-                        if (k_out[v] && k_gap[v] && inside) {
-                            acc_r += rec;
-                            acc_p += (double)po[v];
-                            acc_n += (double)no[v];
-                        }
-                    }
-                    const int64_t row = (int64_t)i * plane + (int64_t)j * nzb;
-                    if constexpr (V == 2) {
-                        if (k_out[0] && k_out[1]) {
-                            *reinterpret_cast<f32x2 *>(PO + row + k0) = f32x2{po[0], po[1]};
-                            *reinterpret_cast<f32x2 *>(NO + row + k0) = f32x2{no[0], no[1]};
+                        const S im_p = bp[PB][r + 1][v], ip_p = bp[XB][r + 1][v];
+                        const S im_n = bn[PB][r + 1][v], ip_n = bn[XB][r + 1][v];
+                        if constexpr (sizeof(S) == 8) {
+                            // the reference's expression, term by term
+                            const A p = (A)cp[r + 1][v], n = (A)cn[r + 1][v];
+                            const A p_new = (A)a.p_zm * (A)p_km[v] + (A)a.p_zp * (A)p_kp[v] +
+                                            (A)a.p_lat * (A)cp[r][v] + (A)a.p_lat * (A)cp[r + 2][v] +
+                                            (A)a.p_lat * (A)im_p + (A)a.p_lat * (A)ip_p + (A)a.p_cen * p;
+                            const A n_new = (A)a.n_zm * (A)n_kp[v] + (A)a.n_zp * (A)n_km[v] +
+                                            (A)a.n_lat * (A)cn[r][v] + (A)a.n_lat * (A)cn[r + 2][v] +
+                                            (A)a.n_lat * (A)im_n + (A)a.n_lat * (A)ip_n + (A)a.n_cen * n;
+                            rec[v] = a.alpha_dt * (double)p * (double)n;
+                            po[v] = (S)((double)p_new - rec[v]);
+                            no[v] = (S)((double)n_new - rec[v]);
                         } else {
-#pragma unroll
-                            for (int v = 0; v < V; ++v)
-                                if (k_out[v]) {
-                                    (PO + row)[k0 + v] = po[v];
-                                    (NO + row)[k0 + v] = no[v];
-                                }
-                        }
-                    } else {
-                        if (k_out[0]) {
-                            (PO + row)[k0] = po[0];
-                            (NO + row)[k0] = no[0];
+                            // FP32 fields: flux form x + sum_w w (x_w - x). The weights
+                            // sum to one, so this is the same update, and written as
+                            // fluxes it moves charge between neighbours without
+                            // creating any, whatever the rounding of the FP32 weights;
+                            // the recombination product stays FP64.
+                            const float p = cp[r + 1][v], n = cn[r + 1][v];
+                            const float tp = f_pzm * (p_km[v] - p) + f_pzp * (p_kp[v] - p) +
+                                             f_plat * (((cp[r][v] - p) + (cp[r + 2][v] - p)) +
+                                                       ((im_p - p) + (ip_p - p)));
+                            const float tn = f_nzm * (n_kp[v] - n) + f_nzp * (n_km[v] - n) +
+                                             f_nlat * (((cn[r][v] - n) + (cn[r + 2][v] - n)) +
+                                                       ((im_n - n) + (ip_n - n)));
+                            rec[v] = a.alpha_dt * (double)p * (double)n;
+                            const float rf = (float)rec[v];
+                            po[v] = (p + tp) - rf;
+                            no[v] = (n + tn) - rf;
                         }
                     }
+                    if (inside) {
+#pragma unroll
+                        for (int v = 0; v < V; ++v) {
+                            sr[v] += rec[v];
+                            sp[v] += (double)po[v];
+                            sn[v] += (double)no[v];
+                        }
+                    }
+                    store2(i, j, po, no);
                 }
             }
+            if (more) add_dep_rows(bp[FB], bn[FB], fd);
+        };
+        using I0 = std::integral_constant<int, 0>;
+        using I1 = std::integral_constant<int, 1>;
+        using I2 = std::integral_constant<int, 2>;
+        using I3 = std::integral_constant<int, 3>;
+        for (int i = i0; i < i1; i += 4) {
+            plane_step(i, I0{}, I1{}, I2{}, I3{});
+            if (i + 1 < i1) plane_step(i + 1, I1{}, I2{}, I3{}, I0{});
+            if (i + 2 < i1) plane_step(i + 2, I2{}, I3{}, I0{}, I1{});
+            if (i + 3 < i1) plane_step(i + 3, I3{}, I0{}, I1{}, I2{});
+        }
 #pragma unroll
-            for (int r = 0; r < TJ; ++r)
-#pragma unroll
-                for (int v = 0; v < V; ++v) {
-                    pp[r][v] = cp[r + 1][v];
-                    pn[r][v] = cn[r + 1][v];
-                }
-#pragma unroll
-            for (int r = 0; r < TJ + 2; ++r)
-#pragma unroll
-                for (int v = 0; v < V; ++v) {
-                    cp[r][v] = xp[r][v];
-                    cn[r][v] = xn[r][v];
-                    xp[r][v] = fp[r][v];
-                    xn[r][v] = fn[r][v];
-                }
+        for (int v = 0; v < V; ++v) {
+            acc_r += k_score[v] ? sr[v] : 0.0;
+            acc_p += k_score[v] ? sp[v] : 0.0;
+            acc_n += k_score[v] ? sn[v] : 0.0;
         }
     }
     block_sum3(acc_r, acc_p, acc_n);
@@ -282,14 +470,17 @@ __global__ void __launch_bounds__(512) sweep_kernel(SweepArgs a) {
 struct GatherArgs {
     const double *xs;
     const double *ys;
-    int64_t m;
-    double *dens;      // no_xy^2, gf-scaled summed density
-    double *partials;  // [gridDim.x] scored injected charge
+    const int64_t *offsets;  // device step offsets into xs/ys (n_steps + 1)
+    int64_t first_step;      // step of blockIdx.y == 0
+    int64_t n_slots;         // step s uses slot s % n_slots of dens and partials
+    double *dens;            // [n_slots][no_xy^2], gf-scaled summed density
+    double *partials;        // [n_slots][gridDim.x], scored injected charge
     int no_xy, no_z, ntj;
     double cutoff, h2b2, gf, mid, rsq;
 };
 
 constexpr int kGT = 16;  // gather tile edge
+constexpr int kGB = 32;  // tracks whose factors are evaluated per LDS batch
 
 __device__ __forceinline__ void track_box(double x, double y, double c, double nx, int &li, int &hi, int &lj,
                                           int &hj) {
@@ -299,33 +490,42 @@ __device__ __forceinline__ void track_box(double x, double y, double c, double n
     hj = (int)fmin(floor(y + c) + 1.0, nx);
 }
 
-// Deposition of one step's tracks as a gather: each thread owns one (i, j)
-// column of a 16x16 tile and sums, in ascending track order, gi(t)*gj(t) over
-// the tracks whose stencil box covers it -- the same terms in the same order
-// as the reference's CSR row accumulation (solver_numba_parallel.py:183-219),
-// so no atomics and a deterministic result. The block first compacts, in
-// order, the tracks whose box meets the tile (ballot + prefix count), then
-// evaluates each listed track's 16 row and 16 column factors once into LDS.
+// Deposition of a block of steps as a gather. Block (x, y) owns a 16x16 tile
+// of columns for step first_step + y; each thread owns one (i, j) column and
+// sums, in ascending track order, gi(t)*gj(t) over the tracks whose stencil
+// box covers it -- the same terms in the same order as the reference's CSR
+// row accumulation (solver_numba_parallel.py:183-219), so no atomics and a
+// deterministic result. The block first compacts, in order, the tracks whose
+// box meets the tile (ballot + prefix count), then evaluates each listed
+// track's 16 row and 16 column factors once into LDS, 32 tracks at a time.
 // Output: dens = sum * gaussian_factor, and per block the scored injected
 // charge sum(dens * no_z) over interior columns inside the scoring disc
-// (solver_numba_parallel.py:250-277).
+// (solver_numba_parallel.py:250-277). Steps without tracks are skipped.
 __global__ void __launch_bounds__(256) gather_kernel(GatherArgs a) {
     __shared__ int s_list[256];
     __shared__ int s_wcnt[4];
-    __shared__ int s_box[8][4];
-    __shared__ double s_g[8][2 * kGT];
+    __shared__ int s_box[kGB][4];
+    __shared__ double s_g[kGB][2 * kGT];
+    const int64_t step = a.first_step + blockIdx.y;
+    const int64_t t_begin = a.offsets[step];
+    const int64_t m = a.offsets[step + 1] - t_begin;
+    if (m <= 0) return;
+    const double *xs = a.xs + t_begin;
+    const double *ys = a.ys + t_begin;
+    const int64_t slot = step % a.n_slots;
+    double *dens = a.dens + slot * a.no_xy * a.no_xy;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int ti0 = (blockIdx.x / a.ntj) * kGT, tj0 = (blockIdx.x % a.ntj) * kGT;
     const int ty = tid / kGT, tx = tid % kGT;
     const int i = ti0 + ty, j = tj0 + tx;
     const double nx = (double)a.no_xy;
     double acc = 0.0;
-    for (int64_t base = 0; base < a.m; base += 256) {
+    for (int64_t base = 0; base < m; base += 256) {
         const int64_t t = base + tid;
         bool hit = false;
-        if (t < a.m) {
+        if (t < m) {
             int li, hi, lj, hj;
-            track_box(a.xs[t], a.ys[t], a.cutoff, nx, li, hi, lj, hj);
+            track_box(xs[t], ys[t], a.cutoff, nx, li, hi, lj, hj);
             hit = li < ti0 + kGT && hi > ti0 && lj < tj0 + kGT && hj > tj0;
         }
         const unsigned long long bal = __ballot(hit);
@@ -339,24 +539,27 @@ __global__ void __launch_bounds__(256) gather_kernel(GatherArgs a) {
         }
         if (hit) s_list[off + before] = tid;
         __syncthreads();
-        for (int sb = 0; sb < nlist; sb += 8) {
-            const int q = tid >> 5, f = tid & 31;
-            if (sb + q < nlist) {
-                const int64_t tt = base + s_list[sb + q];
-                const double x = a.xs[tt], y = a.ys[tt];
-                const double d = f < kGT ? (double)(ti0 + f) - x : (double)(tj0 + f - kGT) - y;
-                s_g[q][f] = exp(-(d * d) * a.h2b2);
-                if (f == 0) {
-                    int li, hi, lj, hj;
-                    track_box(x, y, a.cutoff, nx, li, hi, lj, hj);
-                    s_box[q][0] = li;
-                    s_box[q][1] = hi;
-                    s_box[q][2] = lj;
-                    s_box[q][3] = hj;
+        for (int sb = 0; sb < nlist; sb += kGB) {
+            // 32 tracks x 32 factors: four per thread
+            for (int e = tid; e < kGB * 2 * kGT; e += 256) {
+                const int q = e / (2 * kGT), f = e % (2 * kGT);
+                if (sb + q < nlist) {
+                    const int64_t tt = base + s_list[sb + q];
+                    const double x = xs[tt], y = ys[tt];
+                    const double d = f < kGT ? (double)(ti0 + f) - x : (double)(tj0 + f - kGT) - y;
+                    s_g[q][f] = exp(-(d * d) * a.h2b2);
+                    if (f == 0) {
+                        int li, hi, lj, hj;
+                        track_box(x, y, a.cutoff, nx, li, hi, lj, hj);
+                        s_box[q][0] = li;
+                        s_box[q][1] = hi;
+                        s_box[q][2] = lj;
+                        s_box[q][3] = hj;
+                    }
                 }
             }
             __syncthreads();
-            const int nq = min(8, nlist - sb);
+            const int nq = min(kGB, nlist - sb);
             for (int q2 = 0; q2 < nq; ++q2) {
                 if (i >= s_box[q2][0] && i < s_box[q2][1] && j >= s_box[q2][2] && j < s_box[q2][3])
                     acc += s_g[q2][ty] * s_g[q2][kGT + tx];
@@ -367,13 +570,13 @@ __global__ void __launch_bounds__(256) gather_kernel(GatherArgs a) {
     double inj = 0.0, z1 = 0.0, z2 = 0.0;
     if (i < a.no_xy && j < a.no_xy) {
         const double d = acc * a.gf;
-        a.dens[(int64_t)i * a.no_xy + j] = d;
+        dens[(int64_t)i * a.no_xy + j] = d;
         const double di = (double)i - a.mid, dj = (double)j - a.mid;
         if (acc != 0.0 && i >= 1 && i <= a.no_xy - 2 && j >= 1 && j <= a.no_xy - 2 && di * di + dj * dj < a.rsq)
             inj = d * (double)a.no_z;
     }
     block_sum3(inj, z1, z2);
-    if (tid == 0) a.partials[blockIdx.x] = inj;
+    if (tid == 0) a.partials[slot * gridDim.x + blockIdx.x] = inj;
 }
 
 // Explicit deposition into the current arrays (kernel-level API): add the
@@ -393,19 +596,23 @@ __global__ void __launch_bounds__(256) broadcast_kernel(S *pos, S *neg, const do
     }
 }
 
-// Lateral ring of the next arrays. The sweep writes only the interior, so
-// after it the next buffers' outer ring is stale:
-//  - absorbing: the ring is carried over from the current arrays, with this
-//    step's deposition (which the reference adds to the ring too) included
-//    (state.py:157-176, solver_numba_parallel.py:691-695);
-//  - reflecting: every ring voxel copies the nearest interior column of the
-//    next arrays, i.e. i planes mirrored first, then j planes, corners taking
-//    the diagonal interior column (state.py:113-154).
-template <typename S, bool REFLECT, bool DEP>
-__global__ void __launch_bounds__(256) ring_kernel(S *pos_next, S *neg_next, const S *pos, const S *neg,
-                                                   const double *dens, int no_xy, int nzb, int nze, int no_z) {
-    const int64_t face = (int64_t)no_xy * nzb;           // one i plane
-    const int64_t side = (int64_t)(no_xy - 2) * nzb;     // one j column set, i = 1..no_xy-2
+// Per-step scalars from the block partials of one step, in the same fixed
+// order and compensated summation as the fold the next sweep performs:
+// out4 = (injected, recombined, total+, total-). One block.
+__global__ void __launch_bounds__(256) reduce_kernel(const double *sweep_part, int n_sweep, const double *dep_part,
+                                                     int n_dep, double *out4) {
+    double out[4];
+    block_final4(sweep_part, n_sweep, dep_part, n_dep, out);
+    if (threadIdx.x == 0)
+        for (int v = 0; v < 4; ++v) out4[v] = out[v];
+}
+
+// Reflecting wall, for read-back only: write the lateral ring of an array
+// pair as the copy of the nearest interior column (state.py:113-154).
+template <typename S>
+__global__ void __launch_bounds__(256) ring_mirror_kernel(S *pos, S *neg, int no_xy, int nzb) {
+    const int64_t face = (int64_t)no_xy * nzb;        // one i plane
+    const int64_t side = (int64_t)(no_xy - 2) * nzb;  // j = 0 or no_xy-1, i = 1..no_xy-2
     const int64_t n = 2 * face + 2 * side;
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
         int i, j, k;
@@ -421,72 +628,12 @@ __global__ void __launch_bounds__(256) ring_kernel(S *pos_next, S *neg_next, con
             i = 1 + (int)(q / nzb);
             k = (int)(q % nzb);
         }
-        const int64_t idx = ((int64_t)i * no_xy + j) * nzb + k;
-        if (REFLECT) {
-            const int ci = min(max(i, 1), no_xy - 2), cj = min(max(j, 1), no_xy - 2);
-            const int64_t src = ((int64_t)ci * no_xy + cj) * nzb + k;
-            pos_next[idx] = pos_next[src];
-            neg_next[idx] = neg_next[src];
-        } else {
-            double vp = (double)pos[idx], vn = (double)neg[idx];
-            if (DEP && k >= nze && k < nze + no_z) {
-                const double d = dens[(int64_t)i * no_xy + j];
-                vp = vp + d;
-                vn = vn + d;
-            }
-            pos_next[idx] = (S)vp;
-            neg_next[idx] = (S)vn;
-        }
+        const int ci = min(max(i, 1), no_xy - 2), cj = min(max(j, 1), no_xy - 2);
+        const int64_t dst = ((int64_t)i * no_xy + j) * nzb + k;
+        const int64_t src = ((int64_t)ci * no_xy + cj) * nzb + k;
+        pos[dst] = pos[src];
+        neg[dst] = neg[src];
     }
-}
-
-// Neumaier-compensated accumulation of one term.
-struct Comp {
-    double s = 0.0, c = 0.0;
-    __device__ __forceinline__ void add(double x) {
-        const double t = s + x;
-        c += fabs(s) >= fabs(x) ? (s - t) + x : (x - t) + s;
-        s = t;
-    }
-    __device__ __forceinline__ void add(const Comp &o) {
-        add(o.s);
-        c += o.c;
-    }
-    __device__ __forceinline__ double value() const { return s + c; }
-};
-
-// Per-step scalars from the block partials, in a fixed order with compensated
-// summation: out4 = (injected, recombined, total+, total-). One block.
-__global__ void __launch_bounds__(256) reduce_kernel(const double *sweep_part, int n_sweep, const double *dep_part,
-                                                     int n_dep, double *out4) {
-    __shared__ double s_s[4][256];
-    __shared__ double s_c[4][256];
-    Comp q[4];
-    for (int b = threadIdx.x; b < n_dep; b += 256) q[0].add(dep_part[b]);
-    for (int b = threadIdx.x; b < n_sweep; b += 256) {
-        q[1].add(sweep_part[3 * (int64_t)b + 0]);
-        q[2].add(sweep_part[3 * (int64_t)b + 1]);
-        q[3].add(sweep_part[3 * (int64_t)b + 2]);
-    }
-    for (int v = 0; v < 4; ++v) {
-        s_s[v][threadIdx.x] = q[v].s;
-        s_c[v][threadIdx.x] = q[v].c;
-    }
-    __syncthreads();
-    for (int stride = 128; stride >= 1; stride >>= 1) {
-        if (threadIdx.x < stride) {
-            for (int v = 0; v < 4; ++v) {
-                Comp x{s_s[v][threadIdx.x], s_c[v][threadIdx.x]};
-                Comp y{s_s[v][threadIdx.x + stride], s_c[v][threadIdx.x + stride]};
-                x.add(y);
-                s_s[v][threadIdx.x] = x.s;
-                s_c[v][threadIdx.x] = x.c;
-            }
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0)
-        for (int v = 0; v < 4; ++v) out4[v] = s_s[v][0] + s_c[v][0];
 }
 
 template <typename S>

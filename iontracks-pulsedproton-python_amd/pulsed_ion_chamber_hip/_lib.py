@@ -10,7 +10,11 @@ from pathlib import Path
 
 import numpy as np
 
-LIB_PATH = Path(__file__).resolve().parent / "libpic_hip.so"
+import os
+
+# PIC_HIP_LIBRARY points at an alternative build of the same C ABI (A/B timing
+# of kernel variants); the in-tree library is the default.
+LIB_PATH = Path(os.environ.get("PIC_HIP_LIBRARY", Path(__file__).resolve().parent / "libpic_hip.so"))
 
 PIC_OK = 0
 PIC_E_INVALID = -1
@@ -39,6 +43,7 @@ EXPORTED = (
     "pic_step",
     "pic_read_fields",
     "pic_write_fields",
+    "pic_reset",
     "pic_synchronize",
     "pic_set_tiling",
     "pic_enable_timing",
@@ -111,6 +116,7 @@ def open_library():
         "pic_step": [_VP, _DP],
         "pic_read_fields": [_VP, _DP, _DP],
         "pic_write_fields": [_VP, _DP, _DP],
+        "pic_reset": [_VP],
         "pic_synchronize": [_VP],
         "pic_set_tiling": [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int],
         "pic_enable_timing": [_VP, ctypes.c_int],

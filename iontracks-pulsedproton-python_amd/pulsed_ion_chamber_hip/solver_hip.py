@@ -108,6 +108,9 @@ class HipSimulation:
         _lib.check(self.L.pic_read_per_step(self.handle, int(first), int(count), _lib.dp(out)), "pic_read_per_step")
         return out
 
+    def reset(self) -> None:
+        _lib.check(self.L.pic_reset(self.handle), "pic_reset")
+
     def synchronize(self) -> None:
         _lib.check(self.L.pic_synchronize(self.handle), "pic_synchronize")
 
@@ -127,7 +130,7 @@ class HipSimulation:
         _lib.check(self.L.pic_get_timing(self.handle, ctypes.byref(sweep_ms), ctypes.byref(ns), ctypes.byref(dep_ms),
                                          ctypes.byref(nd), ctypes.byref(other)), "pic_get_timing")
         return dict(sweep_ms=sweep_ms.value, sweep_launches=ns.value, deposit_ms=dep_ms.value,
-                    deposit_launches=nd.value)
+                    deposit_launches=nd.value, ring_reduce_ms=other.value)
 
     # --- kernel level ----------------------------------------------------
     def deposit(self, xs: np.ndarray, ys: np.ndarray) -> float:
