@@ -178,7 +178,9 @@ void record(pic_sim *s, int kind) {
     if (!s->timing || !s->in_run) return;
     if (s->ev_used == s->ev_pool.size()) {
         hipEvent_t e;
-        if (hipEventCreate(&e) != hipSuccess) return;
+        // device-scope release: a timing event must not write back the L2 to
+        // system memory, which would perturb the kernels it brackets
+        if (hipEventCreateWithFlags(&e, hipEventReleaseToDevice) != hipSuccess) return;
         s->ev_pool.push_back(e);
         s->ev_kind.push_back(kind);
     }

@@ -38,6 +38,22 @@ def command(output: Path = OUTPUT):
     ]
 
 
+PROBE_SOURCE = PKG.parent / "csrc" / "hbm_copy_probe.hip"
+PROBE_OUTPUT = PKG / "hbm_copy_probe"
+
+
+def build_probe(verbose: bool = False) -> Path:
+    """The copy-calibration executable used when profiling HBM traffic."""
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-O3", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+           "-o", str(PROBE_OUTPUT), str(PROBE_SOURCE)]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    proc = subprocess.run(cmd, capture_output=True, text=True)
+    if proc.returncode != 0:
+        raise RuntimeError(f"hipcc failed ({proc.returncode}):\n{proc.stderr}")
+    return PROBE_OUTPUT
+
+
 def build(verbose: bool = False) -> Path:
     """Compile unconditionally (a few seconds to a minute)."""
     tmp = OUTPUT.with_suffix(".so.tmp")
@@ -55,4 +71,5 @@ def build(verbose: bool = False) -> Path:
 
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_probe(verbose=True))
     sys.exit(0)
