@@ -14,6 +14,12 @@ FP64 fields is timed too, and the FP32 k_s error against it is reported.
 ``--workload c4`` runs the 5 um full electrode (1066 x 1066 x 410,
 4580 steps, the reference's published A100 headline).
 
+The saturation curve (32 configurations: 8 voltages 50-500 V x 4 dose rates,
+grids 16^2..47^2 x 106..206, 243..2505 steps) is timed as one batch against 32
+sequential plain runs of the same configurations, and its FP32 k_s values are
+compared with an FP64 batch and with the CPU oracle; the default line carries
+it as ``curve``, ``--workload c2`` makes it the headline.
+
 Prints one JSON line. ``value`` is interior voxel-steps per second over the
 timed region; ``roofline`` divides the sweep kernel's compulsory HBM bytes
 (read n+ and n-, write n+' and n-': 16 B per interior voxel in FP32, 32 B in
@@ -115,12 +121,131 @@ def cpu_baseline(cfg, schedule, xs, ys, budget_s):
     }
 
 
+def curve_bench(precision, cpu_budget_s, warmup=20):
+    """The 32-point saturation curve: one batch against 32 sequential plain
+    runs of the same configurations, tracks already resident in HBM for both
+    (the host draws them once, untimed). Returns the curve's figures."""
+    from configs import saturation_curve_configs
+    from pulsed_ion_chamber_hip.batch import HipBatch
+    from pulsed_ion_chamber_hip.pulses import draw_tracks
+    from pulsed_ion_chamber_hip.solver_hip import HipSimulation
+    from pulsed_ion_chamber_hip.state import collection_efficiency
+
+    cfgs = saturation_curve_configs()
+    draws = [draw_tracks(c) for c in cfgs]
+    vsteps = [interior_voxels(c) * c.total_time_steps for c in cfgs]
+
+    def ks_of(sim, c):
+        per = sim.read_per_step(0, c.total_time_steps)
+        return 1.0 / collection_efficiency(per[:, 0], per[:, 1], c.total_time_steps)[-1]
+
+    def batched(prec, timed):
+        sims = [HipSimulation(c, prec) for c in cfgs]
+        for sim, d in zip(sims, draws):
+            sim.upload_tracks(*d)
+        with HipBatch(sims) as b:
+            b.run(0, warmup)
+            b.synchronize()
+            for sim in sims:
+                sim.reset()
+            b.synchronize()
+            t0 = time.perf_counter()
+            b.run()
+            b.synchronize()
+            loop = time.perf_counter() - t0
+            timing = None
+            if timed:
+                # kernel times from a second pass with events around every
+                # launch (the events cost host time, so not in the wall clock)
+                for sim in sims:
+                    sim.reset()
+                b.enable_timing(True)
+                b.run()
+                timing = b.timing()
+            tiling = b.tiling()
+        ks = [ks_of(sim, c) for sim, c in zip(sims, cfgs)]
+        for sim in sims:
+            sim.close()
+        return loop, ks, timing, tiling
+
+    def sequential(prec):
+        total = 0.0
+        ks = []
+        for q, (c, d) in enumerate(zip(cfgs, draws)):
+            with HipSimulation(c, prec) as sim:
+                sim.upload_tracks(*d)
+                if q == 0:
+                    sim.run_steps(0, warmup)
+                    sim.synchronize()
+                    sim.reset()
+                sim.synchronize()
+                t0 = time.perf_counter()
+                for first in range(0, c.total_time_steps, 256):  # run_simulation_hip's chunking
+                    sim.run_steps(first, min(256, c.total_time_steps - first))
+                sim.synchronize()
+                total += time.perf_counter() - t0
+                ks.append(ks_of(sim, c))
+        return total, ks
+
+    loop_b, ks_b, timing, tiling = batched(precision, True)
+    loop_s, ks_s = sequential(precision)
+    _, ks64, _, _ = batched("fp64", False)
+    bytes_per_voxel = 16 if precision == "fp32" else 32
+    sweep_bytes = sum(vsteps) * bytes_per_voxel
+    out = {
+        "configs": len(cfgs),
+        "voxel_steps": sum(vsteps),
+        "global_steps": max(c.total_time_steps for c in cfgs),
+        "grids": f"{min(c.no_xy for c in cfgs)}^2..{max(c.no_xy for c in cfgs)}^2 x "
+                 f"{min(c.no_z_with_buffer for c in cfgs)}..{max(c.no_z_with_buffer for c in cfgs)}",
+        "precision": precision,
+        "batched_s": loop_b,
+        "sequential_plain_s": loop_s,
+        "speedup": loop_s / loop_b,
+        "voxel_steps_per_s": sum(vsteps) / loop_b,
+        "batched_sweep_ms": timing["sweep_ms"],
+        "batched_sweep_launches": timing["sweep_launches"],
+        "batched_sweep_GBs": sweep_bytes / (timing["sweep_ms"] * 1e-3) / 1e9,
+        "gather_ms": timing["gather_ms"],
+        "gather_launches": timing["gather_launches"],
+        "tiling": tiling,
+        "max_dks_vs_fp64": max(abs(a - b) for a, b in zip(ks_b, ks64)),
+        "max_dks_batched_vs_plain": max(abs(a - b) for a, b in zip(ks_b, ks_s)),
+    }
+    if cpu_budget_s > 0:
+        from oracle import batched as oracle_batched
+
+        threads = min(16, len(os.sched_getaffinity(0)))
+        t0 = time.perf_counter()
+        done = 0
+        dks = rel64 = 0.0
+        for c, d, k64 in zip(cfgs, draws, ks64):
+            per, _, _ = oracle_batched.run_per_step(c, *d, nthreads=threads, fields=False)
+            k_ref = 1.0 / collection_efficiency(per[:, 0], per[:, 1], c.total_time_steps)[-1]
+            dks = max(dks, abs(ks_b[done] - k_ref))
+            rel64 = max(rel64, abs(k64 - k_ref) / k_ref)
+            done += 1
+            if time.perf_counter() - t0 > cpu_budget_s:
+                break
+        elapsed = time.perf_counter() - t0
+        out["max_dks_vs_oracle"] = dks
+        out["max_rel_dks_fp64_vs_oracle"] = rel64
+        out["cpu_oracle"] = {
+            "configs": done,
+            "seconds": elapsed,
+            "voxel_steps_per_s": sum(vsteps[:done]) / elapsed,
+            "threads": threads,
+        }
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=0, help="timed steps (default: the whole run)")
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--workload", default="c1", choices=["c0", "c1", "c4"])
+    ap.add_argument("--workload", default="c1", choices=["c0", "c1", "c2", "c4"])
+    ap.add_argument("--no-curve", action="store_true", help="skip the saturation-curve comparison")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
     ap.add_argument("--no-reference-precision", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--cpu-budget-s", type=float, default=12.0)
@@ -136,6 +261,45 @@ def main():
     from pulsed_ion_chamber_hip.pulses import draw_tracks
 
     device, _, _ = _lib.device_info()
+    if args.workload == "c2":
+        print(f"bench: 32-point saturation curve, {args.precision}, device {device}", file=sys.stderr, flush=True)
+        curve = curve_bench(args.precision, 0.0 if args.no_cpu else args.cpu_budget_s, warmup=args.warmup)
+        out = {
+            "metric": METRIC,
+            "value": curve["voxel_steps_per_s"],
+            "unit": "voxel-steps/s",
+            "n_gpus": 1,
+            "steps": curve["global_steps"],
+            "warmup": args.warmup,
+            "ms_per_step": curve["batched_s"] / curve["global_steps"] * 1e3,
+            "higher_is_better": True,
+            "scaling": "weak",
+            "vs_baseline": None,
+            "dtype": "f32" if args.precision == "fp32" else "f64",
+            "data": "synthetic",
+            "config": {"workload": "saturation_curve_32", "grids": curve["grids"], "fields": args.precision},
+            "roofline": {
+                "bound": "hbm",
+                "achieved": curve["batched_sweep_GBs"],
+                "peak": PEAK_GBS,
+                "unit": "GB/s",
+                "frac": curve["batched_sweep_GBs"] / PEAK_GBS,
+                "traffic": None,
+                "kernel": "batch_sweep_kernel",
+                "avg_launch_ms": curve["batched_sweep_ms"] / max(1, curve["batched_sweep_launches"]),
+                "launches": curve["batched_sweep_launches"],
+            },
+            "curve": curve,
+        }
+        if "cpu_oracle" in curve:
+            co = curve["cpu_oracle"]
+            out["cpu_baseline"] = {
+                "value": co["voxel_steps_per_s"], "unit": "voxel-steps/s", "cores": co["threads"], "kind": "port",
+                "sample": f"first {co['configs']} of 32 curve configurations, whole runs, FP64 fields, "
+                          f"{co['threads']} OpenMP threads, {co['seconds']:.1f} s",
+            }
+        print(json.dumps(out))
+        return
     cfg, wl_name = workload(args.workload)
     total = cfg.total_time_steps
     steps = total if args.steps <= 0 else min(args.steps, total)
@@ -208,6 +372,9 @@ def main():
         out["dks_fp32_vs_fp64"] = abs(ks - ks64) if steps == total else None
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cfg, schedule, xs, ys, args.cpu_budget_s)
+    if not args.no_curve and args.workload == "c1":
+        print("bench: saturation curve", file=sys.stderr, flush=True)
+        out["curve"] = curve_bench("fp32", 0.0 if args.no_cpu else args.cpu_budget_s)
     print(json.dumps(out))
 
 

@@ -42,6 +42,7 @@ extern "C" {
 #define PIC_WALL_REFLECTING 1
 
 typedef struct pic_sim pic_sim;
+typedef struct pic_batch pic_batch;
 
 /* Everything the loop reads from SimulationConfig (config.py:180-401,
  * scheme_coefficients config.py:466-482, alpha_dt solver_numba.py:273-274). */
@@ -118,7 +119,7 @@ int pic_reset(pic_sim *sim);
 int pic_synchronize(pic_sim *sim);
 
 /* Sweep launch shape, for tuning: rows per tile (1, 2, 4, 8), planes per
- * block chunk, and k values per lane (1, or 2 for FP32 fields with even nzb);
+ * block chunk, and k values per lane (1, or 2 for FP32 fields);
  * 0 = automatic. Takes effect at the next launch. */
 int pic_set_tiling(pic_sim *sim, int rows_per_tile, int planes_per_chunk, int values_per_lane);
 
@@ -129,6 +130,35 @@ int pic_set_tiling(pic_sim *sim, int rows_per_tile, int planes_per_chunk, int va
 int pic_enable_timing(pic_sim *sim, int enable);
 int pic_get_timing(pic_sim *sim, double *sweep_ms, int64_t *sweep_launches, double *deposit_ms,
                    int64_t *deposit_launches, double *other_ms);
+
+/* Batched runs: many handles (a voltage / dose-rate / seed sweep with uneven
+ * grids and step counts) advanced together, one sweep launch per time step
+ * covering every handle still running. Replaces the reference's answer to a
+ * parameter study -- one process per configuration, each running the loop of
+ * solver_numba_parallel.py:526-723 (docs/PERFORMANCE.md:211-218) -- with one
+ * device stream. The handles must share precision, wall type and device;
+ * each keeps its own arrays, uploaded tracks and per-step record, which
+ * pic_read_per_step / pic_read_fields read as after pic_run_steps. A handle
+ * must not be used on its own while its batch runs; the batch does not own
+ * the handles (destroy it first). */
+int pic_batch_create(pic_sim *const *sims, int n, pic_batch **out);
+int pic_batch_destroy(pic_batch *batch);
+/* Common sweep tiling (rows per tile 1, 2 or 4 -- FP64: 1 or 2 -- and planes
+ * per block chunk; 0 = automatic) and the tiling in use. */
+int pic_batch_set_tiling(pic_batch *batch, int rows_per_tile, int planes_per_chunk);
+int pic_batch_get_tiling(pic_batch *batch, int *rows_per_tile, int *planes_per_chunk, int *values_per_lane,
+                         int *windows_per_block);
+/* Steps [first, first+count) of every handle, each handle stopping at its
+ * own uploaded step count (the loop body of solver_cuda.py:791-865 for all of
+ * them). Asynchronous on the batch's stream; the handles' own streams are
+ * ordered after it. */
+int pic_batch_run(pic_batch *batch, int64_t first, int64_t count);
+int pic_batch_synchronize(pic_batch *batch);
+/* Kernel timing of the last pic_batch_run (as pic_get_timing): summed
+ * milliseconds and launch counts of the batched sweeps and of the gathers. */
+int pic_batch_enable_timing(pic_batch *batch, int enable);
+int pic_batch_get_timing(pic_batch *batch, double *sweep_ms, int64_t *sweep_launches, double *gather_ms,
+                         int64_t *gather_launches);
 
 #ifdef __cplusplus
 }

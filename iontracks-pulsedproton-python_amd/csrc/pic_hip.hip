@@ -10,6 +10,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
+#include <cstdint>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -57,7 +59,8 @@ struct pic_sim {
     int device = 0;
     hipStream_t stream = nullptr;
     size_t elem = 8;
-    int64_t n_vox = 0;
+    int64_t pitch = 0;  // row pitch in elements (nzb, rounded up to even for FP32)
+    int64_t n_vox = 0;  // allocated elements per array, no_xy^2 * pitch
     void *cur_p = nullptr, *cur_n = nullptr, *nxt_p = nullptr, *nxt_n = nullptr;
     double *dens = nullptr;        // [batch + 1 slots][no_xy^2]
     double *sweep_part = nullptr;  // [2 slots][sweep_part_cap][3]
@@ -87,6 +90,35 @@ struct pic_sim {
     bool in_run = false;  // events are recorded inside pic_run_steps only
 };
 
+// Many configurations advanced together: per global step one sweep launch
+// covers the blocks of every configuration still running (their block ranges
+// concatenated), each block reading its configuration's arguments from a
+// device table. The handles keep their own arrays, tracks and records.
+struct pic_batch {
+    std::vector<pic_sim *> sims;  // caller's order
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    int prec = PIC_FP64, v = 1, tj = 0, wpb = 1;
+    bool reflect = false;
+    int force_tj = 0, force_chunk = 0;
+    int64_t waves_wanted = 0;
+    std::vector<int> chunks;                      // candidate planes per block, longest first
+    std::vector<std::vector<SweepShape>> shapes;  // [chunk index][sims[c]]
+    // Per-segment argument tables (a segment is one gather batch of steps),
+    // double-buffered so the host builds segment k+1 while the device runs
+    // segment k: pinned host staging, device copy, and the event that marks
+    // the copy out of a host slot as done.
+    int64_t slot_steps = 0, slot_cfgs = 0;
+    pic::SweepArgs *descs_h[2] = {nullptr, nullptr}, *descs_d[2] = {nullptr, nullptr};
+    int *offs_h[2] = {nullptr, nullptr}, *offs_d[2] = {nullptr, nullptr};
+    hipEvent_t slot_ev[2] = {nullptr, nullptr};
+    bool slot_busy[2] = {false, false};
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;  // pairs around each launch: kind 0 sweep, 1 gather
+    std::vector<int> ev_kind;
+    size_t ev_used = 0;
+};
+
 namespace {
 
 int sm_count(int device) {
@@ -95,14 +127,32 @@ int sm_count(int device) {
     return v;
 }
 
+// k values per lane: FP32 fields take two (8-byte loads; the even row pitch
+// keeps every row 8-byte aligned)
+int values_per_lane(const pic_sim *s, bool force_v1) { return (s->prec == PIC_FP32 && !force_v1) ? 2 : 1; }
+
+// Fixed tiling of one configuration: v values per lane, tj rows per tile,
+// `chunk` planes per block, wpb windows (waves) per block.
+SweepShape fixed_shape(const pic_sim *s, int v, int tj, int chunk, int wpb) {
+    SweepShape sh;
+    sh.v = v;
+    sh.n_windows = (int)ceil_div(s->p.nzb - 2, pic::kWave * v - 2);
+    sh.wpb = wpb;
+    sh.n_kg = (int)ceil_div(sh.n_windows, wpb);
+    const int64_t rows = s->p.no_xy - 2, planes = s->p.no_xy - 2;
+    sh.tj = tj;
+    sh.n_jt = (int)ceil_div(rows, tj);
+    sh.chunk = (int)std::min<int64_t>(chunk, planes);
+    sh.n_ic = (int)ceil_div(planes, sh.chunk);
+    return sh;
+}
+
 // Rows per tile (registers) and planes per block: enough blocks to keep every
 // CU busy, as few halo rows and chunk prologues as that allows.
 SweepShape sweep_shape(const pic_sim *s) {
     SweepShape sh;
     const int64_t no_xy = s->p.no_xy, nzb = s->p.nzb;
-    // FP32 fields take two k values per lane (8-byte loads) when rows start
-    // 8-byte aligned, i.e. when nzb is even
-    sh.v = (s->prec == PIC_FP32 && nzb % 2 == 0 && !s->force_v1) ? 2 : 1;
+    sh.v = values_per_lane(s, s->force_v1);
     sh.n_windows = (int)ceil_div(nzb - 2, pic::kWave * sh.v - 2);
     sh.wpb = std::min(sh.n_windows, 8);
     sh.n_kg = (int)ceil_div(sh.n_windows, sh.wpb);
@@ -188,18 +238,43 @@ void record(pic_sim *s, int kind) {
     if (hipEventRecord(s->ev_pool[s->ev_used], s->stream) == hipSuccess) ++s->ev_used;
 }
 
-// sweep of the current arrays into the next ones, adding the deposition
-// density dens (if dep) and folding the step's scalars into out4
-// Sweep step `step`: adds the deposition density dens (if dep), writes its
-// block partials to slot step % 2 and becomes the pending step whose record
-// out4 is folded later; folds the previously pending step at its start.
-int enqueue_sweep(pic_sim *s, int64_t step, bool dep, const double *dens, const double *dep_part, int n_dep,
-                  double *out4) {
-    const SweepShape sh = sweep_shape(s);
+void record_batch(pic_batch *b, int kind) {
+    if (!b->timing) return;
+    if (b->ev_used == b->ev_pool.size()) {
+        hipEvent_t e;
+        if (hipEventCreateWithFlags(&e, hipEventReleaseToDevice) != hipSuccess) return;
+        b->ev_pool.push_back(e);
+        b->ev_kind.push_back(kind);
+    }
+    b->ev_kind[b->ev_used] = kind;
+    if (hipEventRecord(b->ev_pool[b->ev_used], b->stream) == hipSuccess) ++b->ev_used;
+}
+
+// Grow the sweep partial buffer to hold nblk blocks (a tiling with more
+// blocks than the default capacity); only between runs, no step pending.
+int ensure_partials(pic_sim *s, int64_t nblk) {
+    if (nblk <= s->sweep_part_cap) return PIC_OK;
+    if (s->pending.out4) return fail(PIC_E_INVALID, "cannot grow the partial buffer while a step is pending");
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    double *grown = nullptr;
+    if (hipMalloc(&grown, (size_t)(2 * nblk * 3) * sizeof(double)) != hipSuccess)
+        return fail(PIC_E_NOMEM, "hipMalloc of %lld sweep partials failed", (long long)nblk);
+    if (s->sweep_part) hipFree(s->sweep_part);
+    s->sweep_part = grown;
+    s->sweep_part_cap = nblk;
+    return PIC_OK;
+}
+
+// Arguments of sweep step `step` of one configuration: adds the deposition
+// density dens (if dep), writes its block partials to slot step % 2 and
+// becomes the pending step whose record out4 is folded later; folds the
+// previously pending step at its start. Updates the pending state.
+int make_sweep_args(pic_sim *s, const SweepShape &sh, int64_t step, bool dep, const double *dens,
+                    const double *dep_part, int n_dep, double *out4, pic::SweepArgs &a) {
     const int64_t nblk = sh.blocks();
     if (nblk > s->sweep_part_cap) return fail(PIC_E_INVALID, "sweep grid %lld exceeds partial buffer", (long long)nblk);
     double *part = s->sweep_part + (step % 2) * s->sweep_part_cap * 3;
-    pic::SweepArgs a{};
+    a = pic::SweepArgs{};
     a.pos = s->cur_p;
     a.neg = s->cur_n;
     a.pos_out = s->nxt_p;
@@ -213,6 +288,7 @@ int enqueue_sweep(pic_sim *s, int64_t step, bool dep, const double *dens, const 
     a.prev_out4 = s->pending.out4;
     a.no_xy = (int)s->p.no_xy;
     a.nzb = (int)s->p.nzb;
+    a.pitch = (int)s->pitch;
     a.nze = (int)s->p.nze;
     a.no_z = (int)s->p.no_z;
     a.n_windows = sh.n_windows;
@@ -232,25 +308,62 @@ int enqueue_sweep(pic_sim *s, int64_t step, bool dep, const double *dens, const 
     a.n_zp = s->p.n_zp;
     a.n_cen = s->p.n_cen;
     a.alpha_dt = s->p.alpha_dt;
-    const dim3 grid((unsigned)nblk), block((unsigned)(64 * sh.wpb));
-    const bool reflect = s->p.wall == PIC_WALL_REFLECTING;
-    record(s, 0);
-    int rc = s->prec == PIC_FP64
-                 ? launch_sweep_typed<double, double, 1>(a, sh.tj, grid, block, s->stream, dep, reflect)
-             : sh.v == 2 ? launch_sweep_typed<float, float, 2>(a, sh.tj, grid, block, s->stream, dep, reflect)
-                         : launch_sweep_typed<float, float, 1>(a, sh.tj, grid, block, s->stream, dep, reflect);
-    if (rc) return rc;
-    HIP_LAUNCH_CHECK();
-    record(s, 0);
     s->pending = pic_sim::Pending{part, (int)nblk, dep ? dep_part : nullptr, dep ? n_dep : 0, out4};
     return PIC_OK;
 }
 
+int enqueue_sweep(pic_sim *s, int64_t step, bool dep, const double *dens, const double *dep_part, int n_dep,
+                  double *out4) {
+    const SweepShape sh = sweep_shape(s);
+    pic::SweepArgs a;
+    int rc = make_sweep_args(s, sh, step, dep, dens, dep_part, n_dep, out4, a);
+    if (rc) return rc;
+    // one extra block folds the previous step when one is pending
+    const dim3 grid((unsigned)(sh.blocks() + (a.prev_out4 ? 1 : 0))), block((unsigned)(64 * sh.wpb));
+    const bool reflect = s->p.wall == PIC_WALL_REFLECTING;
+    record(s, 0);
+    rc = s->prec == PIC_FP64 ? launch_sweep_typed<double, double, 1>(a, sh.tj, grid, block, s->stream, dep, reflect)
+         : sh.v == 2         ? launch_sweep_typed<float, float, 2>(a, sh.tj, grid, block, s->stream, dep, reflect)
+                             : launch_sweep_typed<float, float, 1>(a, sh.tj, grid, block, s->stream, dep, reflect);
+    if (rc) return rc;
+    HIP_LAUNCH_CHECK();
+    record(s, 0);
+    return PIC_OK;
+}
+
+template <typename S, typename A, int V>
+int launch_batch_typed(const pic::SweepArgs *descs, const int *offs, int n_cfg, int tj, dim3 grid, dim3 block,
+                       hipStream_t st, bool reflect) {
+#define PIC_BATCH_LAUNCH(TJ_)                                                                                   \
+    do {                                                                                                        \
+        if (reflect)                                                                                            \
+            hipLaunchKernelGGL((pic::batch_sweep_kernel<S, A, TJ_, V, true>), grid, block, 0, st, descs, offs,  \
+                               n_cfg);                                                                          \
+        else                                                                                                    \
+            hipLaunchKernelGGL((pic::batch_sweep_kernel<S, A, TJ_, V, false>), grid, block, 0, st, descs, offs, \
+                               n_cfg);                                                                          \
+    } while (0)
+    switch (tj) {
+        case 1: PIC_BATCH_LAUNCH(1); break;
+        case 2: PIC_BATCH_LAUNCH(2); break;
+        case 4:
+            if constexpr (sizeof(S) == 8) {
+                return fail(PIC_E_INVALID, "batched FP64 sweeps take 1 or 2 rows per tile");
+            } else {
+                PIC_BATCH_LAUNCH(4);
+            }
+            break;
+        default: return fail(PIC_E_INVALID, "unsupported batched rows_per_tile %d", tj);
+    }
+#undef PIC_BATCH_LAUNCH
+    return PIC_OK;
+}
+
 // fold the pending step's partials into its record (end of a run of steps)
-int flush_pending(pic_sim *s) {
+int flush_pending(pic_sim *s, hipStream_t st = nullptr) {
     if (!s->pending.out4) return PIC_OK;
     record(s, 2);
-    hipLaunchKernelGGL(pic::reduce_kernel, dim3(1), dim3(256), 0, s->stream, s->pending.part, s->pending.n,
+    hipLaunchKernelGGL(pic::reduce_kernel, dim3(1), dim3(256), 0, st ? st : s->stream, s->pending.part, s->pending.n,
                        s->pending.dep, s->pending.n_dep, s->pending.out4);
     HIP_LAUNCH_CHECK();
     record(s, 2);
@@ -266,7 +379,7 @@ int64_t gather_blocks(const pic_sim *s) {
 // deposition densities of steps [first, first+count) (count <= batch) into
 // dens[0..count) and dep_part[0..count); offsets are device offsets into xs/ys
 int enqueue_gather(pic_sim *s, const double *xs, const double *ys, const int64_t *offsets, int64_t first,
-                   int64_t count) {
+                   int64_t count, hipStream_t st = nullptr) {
     pic::GatherArgs g{};
     g.xs = xs;
     g.ys = ys;
@@ -284,8 +397,8 @@ int enqueue_gather(pic_sim *s, const double *xs, const double *ys, const int64_t
     g.mid = s->p.mid_xy;
     g.rsq = s->p.scoring_radius_sq;
     record(s, 1);
-    hipLaunchKernelGGL(pic::gather_kernel, dim3((unsigned)gather_blocks(s), (unsigned)count), dim3(256), 0, s->stream,
-                       g);
+    hipLaunchKernelGGL(pic::gather_kernel, dim3((unsigned)gather_blocks(s), (unsigned)count), dim3(256), 0,
+                       st ? st : s->stream, g);
     HIP_LAUNCH_CHECK();
     record(s, 1);
     return PIC_OK;
@@ -301,6 +414,12 @@ int enqueue_deposit_reduce(pic_sim *s, double *out4) {
 void swap_buffers(pic_sim *s) {
     std::swap(s->cur_p, s->nxt_p);
     std::swap(s->cur_n, s->nxt_n);
+}
+
+// elements from one (i, j) row to the next: FP32 rows are padded to an even
+// length so that two-value lanes load 8-byte aligned pairs
+int64_t row_pitch(const pic_params *p, int precision) {
+    return precision == PIC_FP32 ? (p->nzb + 1) / 2 * 2 : p->nzb;
 }
 
 int check_params(const pic_params *p, int precision) {
@@ -326,6 +445,90 @@ size_t partial_cap_bytes(const pic_params *p) {
     const int64_t windows = ceil_div(p->nzb - 2, pic::kOut);
     const int64_t cap = rows * windows * std::max<int64_t>(1, ceil_div(rows, 8)) + 1024;
     return (size_t)cap * 3 * sizeof(double);
+}
+
+// Common tiling of a batch: one block size (wpb waves) and one kernel
+// instance (rows per tile, values per lane) for every configuration, chosen
+// like sweep_shape but for the waves of all configurations together. The
+// planes per block may change from step to step: each step takes the longest
+// chunk that still gives the configurations running at that step enough
+// waves, so the tail of a curve, where one or two grids are left, marches in
+// short chunks.
+int batch_tiling(pic_batch *b) {
+    int v = 2, wpb = 8;
+    for (pic_sim *s : b->sims) v = std::min(v, values_per_lane(s, false));
+    for (pic_sim *s : b->sims)
+        wpb = std::min(wpb, (int)ceil_div(s->p.nzb - 2, pic::kWave * v - 2));
+    const int cus = sm_count(b->sims[0]->device);
+    const int64_t waves_wanted = (int64_t)cus * 16;
+    auto waves = [&](int tj, int chunk) {
+        int64_t w = 0;
+        for (pic_sim *s : b->sims) w += fixed_shape(s, v, tj, chunk, wpb).blocks() * wpb;
+        return w;
+    };
+    int tj = b->force_tj;
+    const int max_tj = b->prec == PIC_FP64 ? 2 : 4;
+    if (tj <= 0) {
+        tj = 1;
+        for (int c = max_tj; c >= 1; c /= 2)
+            if (waves(c, b->force_chunk > 0 ? b->force_chunk : 16) >= waves_wanted) {
+                tj = c;
+                break;
+            }
+    }
+    std::vector<int> chunks;
+    if (b->force_chunk > 0)
+        chunks = {b->force_chunk};
+    else
+        chunks = {16, 8, 4};
+    std::vector<std::vector<SweepShape>> shapes(chunks.size());
+    for (size_t ci = 0; ci < chunks.size(); ++ci)
+        for (pic_sim *s : b->sims) shapes[ci].push_back(fixed_shape(s, v, tj, chunks[ci], wpb));
+    for (size_t c = 0; c < b->sims.size(); ++c) {
+        int64_t most = 0;
+        for (size_t ci = 0; ci < chunks.size(); ++ci) most = std::max(most, shapes[ci][c].blocks());
+        const int rc = ensure_partials(b->sims[c], most);
+        if (rc) return rc;
+    }
+    b->v = v;
+    b->wpb = wpb;
+    b->tj = tj;
+    b->waves_wanted = waves_wanted;
+    b->chunks = chunks;
+    b->shapes = shapes;
+    return PIC_OK;
+}
+
+void free_batch_slots(pic_batch *b) {
+    for (int k = 0; k < 2; ++k) {
+        if (b->descs_h[k]) hipHostFree(b->descs_h[k]);
+        if (b->offs_h[k]) hipHostFree(b->offs_h[k]);
+        if (b->descs_d[k]) hipFree(b->descs_d[k]);
+        if (b->offs_d[k]) hipFree(b->offs_d[k]);
+        b->descs_h[k] = b->descs_d[k] = nullptr;
+        b->offs_h[k] = b->offs_d[k] = nullptr;
+        b->slot_busy[k] = false;
+    }
+    b->slot_steps = b->slot_cfgs = 0;
+}
+
+// argument-table slots for segments of `steps` steps (stream idle)
+int ensure_batch_slots(pic_batch *b, int64_t steps) {
+    const int64_t n = (int64_t)b->sims.size();
+    if (steps <= b->slot_steps && n <= b->slot_cfgs) return PIC_OK;
+    free_batch_slots(b);
+    for (int k = 0; k < 2; ++k) {
+        if (hipHostMalloc((void **)&b->descs_h[k], (size_t)(steps * n) * sizeof(pic::SweepArgs)) != hipSuccess ||
+            hipHostMalloc((void **)&b->offs_h[k], (size_t)(steps * (n + 1)) * sizeof(int)) != hipSuccess ||
+            hipMalloc(&b->descs_d[k], (size_t)(steps * n) * sizeof(pic::SweepArgs)) != hipSuccess ||
+            hipMalloc(&b->offs_d[k], (size_t)(steps * (n + 1)) * sizeof(int)) != hipSuccess) {
+            free_batch_slots(b);
+            return fail(PIC_E_NOMEM, "allocation of the batch argument tables failed");
+        }
+    }
+    b->slot_steps = steps;
+    b->slot_cfgs = n;
+    return PIC_OK;
 }
 
 }  // namespace
@@ -358,7 +561,7 @@ int pic_mem_required(const pic_params *p, int precision, size_t *bytes) {
     if (rc) return rc;
     if (!bytes) return fail(PIC_E_INVALID, "bytes is NULL");
     const size_t elem = precision == PIC_FP64 ? 8 : 4;
-    const size_t vox = (size_t)p->no_xy * p->no_xy * p->nzb;
+    const size_t vox = (size_t)p->no_xy * p->no_xy * row_pitch(p, precision);
     const size_t xy = (size_t)p->no_xy * p->no_xy;
     const size_t gt = (size_t)ceil_div(p->no_xy, pic::kGT);
     const size_t b = (size_t)batch_steps(p) + 1;
@@ -385,7 +588,8 @@ int pic_create(const pic_params *p, int precision, pic_sim **out) {
     s->p = *p;
     s->prec = precision;
     s->elem = precision == PIC_FP64 ? 8 : 4;
-    s->n_vox = p->no_xy * p->no_xy * p->nzb;
+    s->pitch = row_pitch(p, precision);
+    s->n_vox = p->no_xy * p->no_xy * s->pitch;
     HIP_TRY(hipGetDevice(&s->device));
     auto cleanup = [&](int code) {
         pic_destroy(s);
@@ -523,13 +727,13 @@ int pic_deposit(pic_sim *s, const double *xs, const double *ys, int64_t m, doubl
         if (!rc) {
             const int64_t n = s->p.no_xy * s->p.no_xy * s->p.no_z;
             const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, 256), 8192));
-            const int no_xy = (int)s->p.no_xy, nzb = (int)s->p.nzb, nze = (int)s->p.nze, no_z = (int)s->p.no_z;
+            const int no_xy = (int)s->p.no_xy, pitch = (int)s->pitch, nze = (int)s->p.nze, no_z = (int)s->p.no_z;
             if (s->prec == PIC_FP64)
                 hipLaunchKernelGGL(pic::broadcast_kernel<double>, dim3(blocks), dim3(256), 0, s->stream,
-                                   (double *)s->cur_p, (double *)s->cur_n, s->dens, no_xy, nzb, nze, no_z);
+                                   (double *)s->cur_p, (double *)s->cur_n, s->dens, no_xy, pitch, nze, no_z);
             else
                 hipLaunchKernelGGL(pic::broadcast_kernel<float>, dim3(blocks), dim3(256), 0, s->stream,
-                                   (float *)s->cur_p, (float *)s->cur_n, s->dens, no_xy, nzb, nze, no_z);
+                                   (float *)s->cur_p, (float *)s->cur_n, s->dens, no_xy, pitch, nze, no_z);
             if (hipGetLastError() != hipSuccess) rc = fail(PIC_E_HIP, "broadcast launch failed");
         }
         if (!rc) rc = enqueue_deposit_reduce(s, s->scratch4);
@@ -568,14 +772,14 @@ int pic_read_fields(pic_sim *s, double *pos, double *neg) {
     if (!s) return fail(PIC_E_INVALID, "sim is NULL");
     if (s->p.wall == PIC_WALL_REFLECTING) {
         // the sweep reads the reflecting ring through the interior; store it now
-        const int64_t n = (4 * s->p.no_xy - 4) * s->p.nzb;
+        const int64_t n = (4 * s->p.no_xy - 4) * s->pitch;
         const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, 256), 4096));
         if (s->prec == PIC_FP64)
             hipLaunchKernelGGL(pic::ring_mirror_kernel<double>, dim3(blocks), dim3(256), 0, s->stream,
-                               (double *)s->cur_p, (double *)s->cur_n, (int)s->p.no_xy, (int)s->p.nzb);
+                               (double *)s->cur_p, (double *)s->cur_n, (int)s->p.no_xy, (int)s->pitch);
         else
             hipLaunchKernelGGL(pic::ring_mirror_kernel<float>, dim3(blocks), dim3(256), 0, s->stream,
-                               (float *)s->cur_p, (float *)s->cur_n, (int)s->p.no_xy, (int)s->p.nzb);
+                               (float *)s->cur_p, (float *)s->cur_n, (int)s->p.no_xy, (int)s->pitch);
         HIP_LAUNCH_CHECK();
     }
     HIP_TRY(hipStreamSynchronize(s->stream));
@@ -589,7 +793,9 @@ int pic_read_fields(pic_sim *s, double *pos, double *neg) {
         } else {
             tmp.resize(s->n_vox);
             HIP_TRY(hipMemcpy(tmp.data(), src[q], s->n_vox * 4, hipMemcpyDeviceToHost));
-            for (int64_t e = 0; e < s->n_vox; ++e) dst[q][e] = (double)tmp[e];
+            const int64_t rows = s->p.no_xy * s->p.no_xy, nzb = s->p.nzb;
+            for (int64_t r = 0; r < rows; ++r)
+                for (int64_t k = 0; k < nzb; ++k) dst[q][r * nzb + k] = (double)tmp[r * s->pitch + k];
         }
     }
     return PIC_OK;
@@ -606,8 +812,10 @@ int pic_write_fields(pic_sim *s, const double *pos, const double *neg) {
             if (s->prec == PIC_FP64) {
                 HIP_TRY(hipMemcpy(dst[q][b], src[q], s->n_vox * 8, hipMemcpyHostToDevice));
             } else {
-                tmp.resize(s->n_vox);
-                for (int64_t e = 0; e < s->n_vox; ++e) tmp[e] = (float)src[q][e];
+                tmp.assign(s->n_vox, 0.0f);  // row padding stays zero
+                const int64_t rows = s->p.no_xy * s->p.no_xy, nzb = s->p.nzb;
+                for (int64_t r = 0; r < rows; ++r)
+                    for (int64_t k = 0; k < nzb; ++k) tmp[r * s->pitch + k] = (float)src[q][r * nzb + k];
                 HIP_TRY(hipMemcpy(dst[q][b], tmp.data(), s->n_vox * 4, hipMemcpyHostToDevice));
             }
         }
@@ -625,21 +833,24 @@ int pic_set_tiling(pic_sim *s, int rows_per_tile, int planes_per_chunk, int valu
     if (!s) return fail(PIC_E_INVALID, "sim is NULL");
     if (rows_per_tile < 0 || planes_per_chunk < 0) return fail(PIC_E_INVALID, "negative tiling");
     if (values_per_lane < 0 || values_per_lane > 2) return fail(PIC_E_INVALID, "values_per_lane must be 0, 1 or 2");
-    if (values_per_lane == 2 && (s->prec != PIC_FP32 || s->p.nzb % 2))
-        return fail(PIC_E_INVALID, "two values per lane needs FP32 fields and an even nzb");
-    s->force_v1 = values_per_lane == 1;
+    if (values_per_lane == 2 && s->prec != PIC_FP32)
+        return fail(PIC_E_INVALID, "two values per lane needs FP32 fields");
     if (rows_per_tile && rows_per_tile != 1 && rows_per_tile != 2 && rows_per_tile != 4 && rows_per_tile != 8)
         return fail(PIC_E_INVALID, "rows_per_tile must be 0, 1, 2, 4 or 8");
     if (rows_per_tile == 8 && (s->prec != PIC_FP32 || values_per_lane != 1))
         return fail(PIC_E_INVALID, "rows_per_tile 8 needs FP32 fields and values_per_lane 1");
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    const int old_tj = s->force_tj, old_chunk = s->force_chunk, old_v1 = s->force_v1;
+    s->force_v1 = values_per_lane == 1;
     s->force_tj = rows_per_tile;
     s->force_chunk = planes_per_chunk;
-    const SweepShape sh = sweep_shape(s);
-    if (sh.blocks() > s->sweep_part_cap) {
-        s->force_tj = s->force_chunk = s->force_v1 = 0;
-        return fail(PIC_E_INVALID, "tiling needs %lld blocks, more than the partial buffer holds", (long long)sh.blocks());
+    const int rc = ensure_partials(s, sweep_shape(s).blocks());
+    if (rc) {
+        s->force_tj = old_tj;
+        s->force_chunk = old_chunk;
+        s->force_v1 = old_v1;
     }
-    return PIC_OK;
+    return rc;
 }
 
 int pic_enable_timing(pic_sim *s, int enable) {
@@ -688,6 +899,216 @@ int pic_reset(pic_sim *s) {
         HIP_LAUNCH_CHECK();
     }
     HIP_TRY(hipStreamSynchronize(s->stream));
+    return PIC_OK;
+}
+
+int pic_batch_create(pic_sim *const *sims, int n, pic_batch **out) {
+    if (!out) return fail(PIC_E_INVALID, "out is NULL");
+    *out = nullptr;
+    if (!sims || n < 1) return fail(PIC_E_INVALID, "a batch needs at least one handle");
+    for (int c = 0; c < n; ++c) {
+        if (!sims[c]) return fail(PIC_E_INVALID, "handle %d is NULL", c);
+        for (int d = 0; d < c; ++d)
+            if (sims[d] == sims[c]) return fail(PIC_E_INVALID, "handle %d appears twice", c);
+        if (sims[c]->prec != sims[0]->prec) return fail(PIC_E_INVALID, "handles of one batch share a precision");
+        if (sims[c]->p.wall != sims[0]->p.wall) return fail(PIC_E_INVALID, "handles of one batch share a wall type");
+        if (sims[c]->device != sims[0]->device) return fail(PIC_E_INVALID, "handles of one batch share a device");
+    }
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev != sims[0]->device) return fail(PIC_E_INVALID, "the handles live on device %d, current is %d", sims[0]->device, dev);
+    pic_batch *b = new pic_batch();
+    b->sims.assign(sims, sims + n);
+    b->prec = sims[0]->prec;
+    b->reflect = sims[0]->p.wall == PIC_WALL_REFLECTING;
+    auto cleanup = [&](int code) {
+        pic_batch_destroy(b);
+        return code;
+    };
+    int rc = batch_tiling(b);
+    if (rc) return cleanup(rc);
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&b->done, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->slot_ev[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->slot_ev[1], hipEventDisableTiming) != hipSuccess)
+        return cleanup(fail(PIC_E_HIP, "stream or event creation failed"));
+    *out = b;
+    return PIC_OK;
+}
+
+int pic_batch_destroy(pic_batch *b) {
+    if (!b) return PIC_OK;
+    if (b->stream) hipStreamSynchronize(b->stream);
+    free_batch_slots(b);
+    for (hipEvent_t e : b->slot_ev)
+        if (e) hipEventDestroy(e);
+    if (b->done) hipEventDestroy(b->done);
+    for (hipEvent_t e : b->ev_pool) hipEventDestroy(e);
+    if (b->stream) hipStreamDestroy(b->stream);
+    delete b;
+    return PIC_OK;
+}
+
+int pic_batch_set_tiling(pic_batch *b, int rows_per_tile, int planes_per_chunk) {
+    if (!b) return fail(PIC_E_INVALID, "batch is NULL");
+    if (rows_per_tile != 0 && rows_per_tile != 1 && rows_per_tile != 2 && rows_per_tile != 4)
+        return fail(PIC_E_INVALID, "batched rows_per_tile must be 0, 1, 2 or 4");
+    if (rows_per_tile == 4 && b->prec == PIC_FP64) return fail(PIC_E_INVALID, "batched FP64 sweeps take 1 or 2 rows per tile");
+    if (planes_per_chunk < 0) return fail(PIC_E_INVALID, "negative planes_per_chunk");
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    const int old_tj = b->force_tj, old_chunk = b->force_chunk;
+    b->force_tj = rows_per_tile;
+    b->force_chunk = planes_per_chunk;
+    int rc = batch_tiling(b);
+    if (rc) {
+        b->force_tj = old_tj;
+        b->force_chunk = old_chunk;
+        batch_tiling(b);
+    }
+    return rc;
+}
+
+int pic_batch_get_tiling(pic_batch *b, int *rows_per_tile, int *planes_per_chunk, int *values_per_lane,
+                         int *windows_per_block) {
+    if (!b) return fail(PIC_E_INVALID, "batch is NULL");
+    if (rows_per_tile) *rows_per_tile = b->tj;
+    if (planes_per_chunk) *planes_per_chunk = b->chunks.empty() ? 0 : b->chunks[0];
+    if (values_per_lane) *values_per_lane = b->v;
+    if (windows_per_block) *windows_per_block = b->wpb;
+    return PIC_OK;
+}
+
+int pic_batch_run(pic_batch *b, int64_t first, int64_t count) {
+    if (!b) return fail(PIC_E_INVALID, "batch is NULL");
+    const int n = (int)b->sims.size();
+    int64_t longest = 0;
+    for (pic_sim *s : b->sims) longest = std::max(longest, s->n_steps);
+    if (first < 0 || count < 0 || first + count > longest)
+        return fail(PIC_E_INVALID, "steps [%lld, %lld) outside the longest uploaded run (%lld)", (long long)first,
+                    (long long)(first + count), (long long)longest);
+    // run order: longest first, so the configurations still running at any
+    // step are a prefix and the grid covers exactly their blocks
+    std::vector<int> ord(n);
+    for (int c = 0; c < n; ++c) ord[c] = c;
+    std::stable_sort(ord.begin(), ord.end(),
+                     [&](int x, int y) { return b->sims[x]->n_steps > b->sims[y]->n_steps; });
+    int64_t gbatch = 128;
+    for (pic_sim *s : b->sims) gbatch = std::min(gbatch, s->batch);
+    // every handle's queued work is done and nothing still reads the tables
+    for (pic_sim *s : b->sims) HIP_TRY(hipStreamSynchronize(s->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    b->slot_busy[0] = b->slot_busy[1] = false;
+    int rc = ensure_batch_slots(b, gbatch);
+    if (rc) return rc;
+    for (pic_sim *s : b->sims) s->pending = pic_sim::Pending{};  // earlier runs were flushed
+    b->ev_used = 0;
+    const dim3 block((unsigned)(64 * b->wpb));
+    std::vector<int> active, chunk_of;
+    int k = 0;
+    // one segment per gather batch: gathers first, then the steps' sweeps
+    for (int64_t s0 = first; s0 < first + count && rc == PIC_OK; s0 += gbatch, k ^= 1) {
+        const int64_t s1 = std::min(first + count, s0 + gbatch);
+        if (b->slot_busy[k]) HIP_TRY(hipEventSynchronize(b->slot_ev[k]));  // host slot k copied out
+        pic::SweepArgs *descs = b->descs_h[k];
+        int *offs = b->offs_h[k];
+        active.assign(s1 - s0, 0);
+        // descriptors of every (step, configuration) of the segment, built in
+        // step order so each handle's buffer swaps and pending steps advance
+        for (int64_t t = s0; t < s1 && rc == PIC_OK; ++t) {
+            int na = 0;
+            while (na < n && t < b->sims[ord[na]]->n_steps) ++na;
+            size_t ci = 0;  // longest chunk that keeps the device busy
+            for (; ci + 1 < b->chunks.size(); ++ci) {
+                int64_t w = 0;
+                for (int q = 0; q < na; ++q) w += b->shapes[ci][ord[q]].blocks() * b->wpb;
+                if (w >= b->waves_wanted) break;
+            }
+            int *o = offs + (t - s0) * (n + 1);
+            o[0] = 0;
+            for (int q = 0; q < na && rc == PIC_OK; ++q) {
+                pic_sim *s = b->sims[ord[q]];
+                const SweepShape &sh = b->shapes[ci][ord[q]];
+                const int n_tiles = (int)gather_blocks(s);
+                const int64_t xy = s->p.no_xy * s->p.no_xy;
+                const int64_t slot = t % (s->batch + 1);
+                const bool dep = s->offsets[t + 1] > s->offsets[t];
+                rc = make_sweep_args(s, sh, t, dep, s->dens + slot * xy, s->dep_part + slot * n_tiles, n_tiles,
+                                     s->per_step + 4 * t, descs[(t - s0) * n + q]);
+                swap_buffers(s);
+                const int64_t nxt = (int64_t)o[q] + sh.blocks();
+                if (nxt > INT32_MAX / 2) rc = fail(PIC_E_INVALID, "batch grid too large");
+                o[q + 1] = (int)nxt;
+            }
+            active[t - s0] = na;
+        }
+        if (rc) break;
+        HIP_TRY(hipMemcpyAsync(b->descs_d[k], descs, (size_t)((s1 - s0) * n) * sizeof(pic::SweepArgs),
+                               hipMemcpyHostToDevice, b->stream));
+        HIP_TRY(hipMemcpyAsync(b->offs_d[k], offs, (size_t)((s1 - s0) * (n + 1)) * sizeof(int),
+                               hipMemcpyHostToDevice, b->stream));
+        HIP_TRY(hipEventRecord(b->slot_ev[k], b->stream));
+        b->slot_busy[k] = true;
+        for (int q = 0; q < n && rc == PIC_OK; ++q) {
+            pic_sim *s = b->sims[ord[q]];
+            const int64_t e = std::min(s1, s->n_steps);
+            if (s0 >= e) break;
+            if (s->offsets[e] > s->offsets[s0]) {
+                record_batch(b, 1);
+                rc = enqueue_gather(s, s->xs, s->ys, s->offsets_d, s0, e - s0, b->stream);
+                record_batch(b, 1);
+            }
+        }
+        for (int64_t t = s0; t < s1 && rc == PIC_OK; ++t) {
+            const int na = active[t - s0];
+            const int *o = offs + (t - s0) * (n + 1);
+            const dim3 grid((unsigned)(o[na] + na));  // na fold blocks, then the sweep blocks
+            const pic::SweepArgs *d = b->descs_d[k] + (t - s0) * n;
+            const int *od = b->offs_d[k] + (t - s0) * (n + 1);
+            record_batch(b, 0);
+            rc = b->prec == PIC_FP64 ? launch_batch_typed<double, double, 1>(d, od, na, b->tj, grid, block, b->stream, b->reflect)
+                 : b->v == 2         ? launch_batch_typed<float, float, 2>(d, od, na, b->tj, grid, block, b->stream, b->reflect)
+                                     : launch_batch_typed<float, float, 1>(d, od, na, b->tj, grid, block, b->stream, b->reflect);
+            if (!rc && hipGetLastError() != hipSuccess) rc = fail(PIC_E_HIP, "batched sweep launch failed");
+            record_batch(b, 0);
+        }
+    }
+    for (pic_sim *s : b->sims)
+        if (!rc) rc = flush_pending(s, b->stream);
+    // later calls on a handle's own stream wait for the batch
+    if (hipEventRecord(b->done, b->stream) != hipSuccess) return rc ? rc : fail(PIC_E_HIP, "hipEventRecord failed");
+    for (pic_sim *s : b->sims)
+        if (hipStreamWaitEvent(s->stream, b->done, 0) != hipSuccess && !rc) rc = fail(PIC_E_HIP, "hipStreamWaitEvent failed");
+    return rc;
+}
+
+int pic_batch_enable_timing(pic_batch *b, int enable) {
+    if (!b) return fail(PIC_E_INVALID, "batch is NULL");
+    b->timing = enable != 0;
+    return PIC_OK;
+}
+
+int pic_batch_get_timing(pic_batch *b, double *sweep_ms, int64_t *sweep_launches, double *gather_ms,
+                         int64_t *gather_launches) {
+    if (!b) return fail(PIC_E_INVALID, "batch is NULL");
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    double t[2] = {0, 0};
+    int64_t n[2] = {0, 0};
+    for (size_t q = 0; q + 1 < b->ev_used; q += 2) {
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, b->ev_pool[q], b->ev_pool[q + 1]));
+        t[b->ev_kind[q]] += ms;
+        ++n[b->ev_kind[q]];
+    }
+    if (sweep_ms) *sweep_ms = t[0];
+    if (sweep_launches) *sweep_launches = n[0];
+    if (gather_ms) *gather_ms = t[1];
+    if (gather_launches) *gather_launches = n[1];
+    return PIC_OK;
+}
+
+int pic_batch_synchronize(pic_batch *b) {
+    if (!b) return fail(PIC_E_INVALID, "batch is NULL");
+    HIP_TRY(hipStreamSynchronize(b->stream));
     return PIC_OK;
 }
 

@@ -1,12 +1,16 @@
 // Device kernels of the gfx950 backend. Included once, by pic_hip.hip.
 //
 // Grid layout: each species is (no_xy, no_xy, nzb), k fastest, stored as S
-// (double or float). Interior voxels are 1 <= i,j <= no_xy-2, 1 <= k <= nzb-2.
+// (double or float) with a row pitch of `pitch` >= nzb elements (FP32 rows are
+// padded to an even length so every row starts 8-byte aligned; the padding is
+// zero and never written). Interior voxels are 1 <= i,j <= no_xy-2,
+// 1 <= k <= nzb-2.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #include <type_traits>
+#include <utility>
 
 namespace pic {
 
@@ -22,6 +26,26 @@ __device__ __forceinline__ int xcd_logical_block(int b, int nblk) {
     const int xcd = b % kXcds, slot = b / kXcds;
     const int start = xcd < rem ? xcd * (per + 1) : rem * (per + 1) + (xcd - rem) * per;
     return start + slot;
+}
+
+// Call f(std::integral_constant<int, q>) for q = 0 .. N-1 (compile-time loop).
+template <typename F, int... Q>
+__device__ __forceinline__ void static_for_impl(F &&f, std::integer_sequence<int, Q...>) {
+    (f(std::integral_constant<int, Q>{}), ...);
+}
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F &&f) {
+    static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+// Planes a sweep wave keeps in flight ahead of the one it computes. Each
+// buffered plane costs 2 (TJ + 2) V registers. Measured on MI355X: one plane
+// ahead (its load issued a full step before its first use) is best for every
+// tile shape; two to four cost occupancy and gained nothing, from the 16^2 x
+// 106 grids of a saturation curve to the 1066^2 x 410 full electrode.
+template <typename S, int TJ, int V>
+constexpr int lookahead_planes() {
+    return 1;
 }
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -82,17 +106,43 @@ __device__ __forceinline__ Comp wave_comp(Comp x) {
 }
 
 // Sum a [n3][3] table of block partials and, if given, an [n_dep] table,
-// over the whole block in a fixed order with compensated
-// summation; thread 0 receives the four totals. Used by the last block of a
-// sweep to turn block partials into the step's scalars without another launch.
+// over the whole block in a fixed order with compensated summation; thread 0
+// receives the four totals. Used by the fold block of the next sweep (and by
+// reduce_kernel) to turn block partials into the step's scalars.
 __device__ void block_final4(const double *part3, int n3, const double *dep, int n_dep, double out[4]) {
     __shared__ double s_s[4][16], s_c[4][16];
     Comp q[4];
-    for (int b = threadIdx.x; b < n_dep; b += blockDim.x) q[0].add(dep[b]);
-    for (int b = threadIdx.x; b < n3; b += blockDim.x) {
-        q[1].add(part3[3 * (int64_t)b + 0]);
-        q[2].add(part3[3 * (int64_t)b + 1]);
-        q[3].add(part3[3 * (int64_t)b + 2]);
+    // Thread t adds entries t, t + B, t + 2B, ... (B = blockDim.x) in that
+    // order. The loads of U entries are issued together before the adds (index
+    // clamped, so no load sits behind a branch): a compensated sum is a
+    // dependent chain, and loading inside it would cost one memory latency
+    // per entry.
+    constexpr int U = 8;
+    const int B = blockDim.x;
+    for (int b0 = threadIdx.x; b0 < n_dep; b0 += U * B) {
+        double v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = dep[min(b0 + u * B, n_dep - 1)];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (b0 + u * B < n_dep) q[0].add(v[u]);
+    }
+    for (int b0 = threadIdx.x; b0 < n3; b0 += U * B) {
+        double v[U][3];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t b = min(b0 + u * B, n3 - 1);
+            v[u][0] = part3[3 * b + 0];
+            v[u][1] = part3[3 * b + 1];
+            v[u][2] = part3[3 * b + 2];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (b0 + u * B < n3) {
+                q[1].add(v[u][0]);
+                q[2].add(v[u][1]);
+                q[3].add(v[u][2]);
+            }
     }
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
@@ -121,13 +171,14 @@ struct SweepArgs {
     void *neg_out;
     const double *dens;      // gf-scaled 2D density of this step, or null
     double *partials;        // [gridDim.x][3]: recombined, total+, total-
-    // the previous step's partials, folded into its record by block 0
+    // the previous step's partials, folded into its record by a block of its own
     const double *prev_part;      // [prev_n][3], or null
     int prev_n;
     const double *prev_dep_part;  // [prev_n_dep], or null
     int prev_n_dep;
     double *prev_out4;            // (injected, recombined, total+, total-), or null
     int no_xy, nzb, nze, no_z;
+    int pitch;      // elements from one (i, j) row to the next
     int n_windows;  // k windows of kOut outputs covering k = 1..nzb-2
     int wpb;        // windows (waves) per block
     int n_kg, n_jt, n_ic;
@@ -173,11 +224,11 @@ typedef float f32x2 __attribute__((ext_vector_type(2)));
 //    (state.py:157-176, solver_numba_parallel.py:691-695).
 // The k end planes are zero in both buffers and are never written.
 //
-// Reduction: each block writes its three scored sums to its slot; block 0 of
-// the NEXT step's launch folds them (with that step's deposition partials) in
-// a fixed order with compensated FP64 summation into the step's record, so
-// the kernel boundary orders the partials and no device-wide fence is needed.
-// The last step of a run is folded by reduce_kernel.
+// Reduction: each block writes its three scored sums to its slot; a block of
+// its own in the NEXT step's launch folds them (with that step's deposition
+// partials) in a fixed order with compensated FP64 summation into the step's
+// record, so the kernel boundary orders the partials and no device-wide fence
+// is needed. The last step of a run is folded by reduce_kernel.
 //
 // The arithmetic is the reference's, term for term and in the same order
 // (solver_numba_parallel.py:374-405); with S = A = double and contraction off
@@ -187,11 +238,14 @@ typedef float f32x2 __attribute__((ext_vector_type(2)));
 // charge -- the weights no longer sum to one -- and that per-step drift, not
 // the FP32 rounding of each value, is what would move k_s. The recombination
 // product and every scored sum stay FP64.
-template <typename S, typename A, int TJ, bool DEP, int V, bool REFLECT>
-__global__ void __launch_bounds__(512) sweep_kernel(SweepArgs a) {
+//
+// sweep_body runs one block's share: `logical` is the block's index among the
+// `nblk` blocks of this configuration (already XCD-remapped), DEP says whether
+// this step deposits (a runtime flag in the multi-configuration kernel).
+template <typename S, typename A, int TJ, int V, bool REFLECT>
+__device__ __forceinline__ void sweep_body(const SweepArgs &a, const int logical, const bool DEP) {
     static_assert(V == 1 || (V == 2 && sizeof(S) == 4), "two values per lane is the FP32 layout");
     constexpr int WIN = kWave * V - 2;  // outputs per window
-    const int logical = xcd_logical_block(blockIdx.x, gridDim.x);
     const int kg = logical % a.n_kg;
     const int rest = logical / a.n_kg;
     const int jt = rest % a.n_jt;
@@ -201,18 +255,11 @@ __global__ void __launch_bounds__(512) sweep_kernel(SweepArgs a) {
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int win = kg * a.wpb + wave;
 
-    if (blockIdx.x == 0 && a.prev_out4) {
-        double out[4];
-        block_final4(a.prev_part, a.prev_n, a.prev_dep_part, a.prev_n_dep, out);
-        if (threadIdx.x == 0)
-            for (int v = 0; v < 4; ++v) a.prev_out4[v] = out[v];
-    }
-
     double acc_r = 0.0, acc_p = 0.0, acc_n = 0.0;
-    if (win < a.n_windows) {
-        const int no_xy = a.no_xy, nzb = a.nzb;
+    if (wave < a.wpb && win < a.n_windows) {
+        const int no_xy = a.no_xy, nzb = a.nzb, pitch = a.pitch;
         const int k0 = win * WIN + V * lane;
-        const bool k_in = k0 < nzb;  // with V = 2, nzb is even so k0+1 < nzb too
+        const bool k_in = k0 < nzb;  // with V = 2, k0 is even and k0+1 < pitch
         bool k_out[V], k_gap[V], k_score[V];
 #pragma unroll
         for (int v = 0; v < V; ++v) {
@@ -233,31 +280,37 @@ __global__ void __launch_bounds__(512) sweep_kernel(SweepArgs a) {
         S *__restrict__ PO = static_cast<S *>(a.pos_out);
         S *__restrict__ NO = static_cast<S *>(a.neg_out);
         const double *__restrict__ dens = a.dens;
-        const int64_t plane = (int64_t)no_xy * nzb;
+        const int64_t plane = (int64_t)no_xy * pitch;
         const float f_pzm = (float)a.p_zm, f_pzp = (float)a.p_zp, f_plat = (float)a.p_lat;
         const float f_nzm = (float)a.n_zm, f_nzp = (float)a.n_zp, f_nlat = (float)a.n_lat;
 
         // raw values of (i, j, k0..k0+V-1); the row base is wave-uniform, so
         // each load is one scalar base plus the lane's k offset. Under a
         // reflecting wall a ring row reads its mirror column.
+        //
+        // No load of the march sits behind a branch: lanes past the end of the
+        // row read its last values instead (they only ever feed lanes that
+        // store nothing and sums that are masked), and callers clamp rows and
+        // planes into the grid. With straight-line loads the compiler knows
+        // exactly how many memory operations were issued after each one, so a
+        // wave waits only for the plane it is about to use while the planes
+        // behind it stay in flight; a conditional load would force it to drain
+        // every load at each use.
+        const int kc = k_in ? k0 : pitch - V;
         auto load_raw = [&](int i, int j, S (&vp)[V], S (&vn)[V]) {
             const int si = REFLECT ? min(max(i, 1), no_xy - 2) : i;
             const int sj = REFLECT ? min(max(j, 1), no_xy - 2) : j;
-            const int64_t row = (int64_t)si * plane + (int64_t)sj * nzb;
-#pragma unroll
-            for (int v = 0; v < V; ++v) vp[v] = vn[v] = (S)0;
-            if (k_in) {
-                if constexpr (V == 2) {
-                    const f32x2 tp = *reinterpret_cast<const f32x2 *>(P + row + k0);
-                    const f32x2 tn = *reinterpret_cast<const f32x2 *>(N + row + k0);
-                    vp[0] = tp.x;
-                    vp[1] = tp.y;
-                    vn[0] = tn.x;
-                    vn[1] = tn.y;
-                } else {
-                    vp[0] = (P + row)[k0];
-                    vn[0] = (N + row)[k0];
-                }
+            const int64_t row = (int64_t)si * plane + (int64_t)sj * pitch;
+            if constexpr (V == 2) {
+                const f32x2 tp = *reinterpret_cast<const f32x2 *>(P + row + kc);
+                const f32x2 tn = *reinterpret_cast<const f32x2 *>(N + row + kc);
+                vp[0] = tp.x;
+                vp[1] = tp.y;
+                vn[0] = tn.x;
+                vn[1] = tn.y;
+            } else {
+                vp[0] = (P + row)[kc];
+                vn[0] = (N + row)[kc];
             }
         };
         // This step's deposition of the TJ+2 rows of a plane: one vector load in
@@ -294,25 +347,21 @@ __global__ void __launch_bounds__(512) sweep_kernel(SweepArgs a) {
         };
         // rows j0-1 .. j0+nrows of plane i; the deposition is added later, when
         // the plane is consumed, so the loads stay in flight meanwhile
+        // (rows past the tile's last halo row repeat it and are never used)
         auto load_rows = [&](int i, S (&rp)[TJ + 2][V], S (&rn)[TJ + 2][V], double &rd) {
 #pragma unroll
-            for (int r = 0; r < TJ + 2; ++r) {
-#pragma unroll
-                for (int v = 0; v < V; ++v) rp[r][v] = rn[r][v] = (S)0;
-                if (r <= nrows + 1) load_raw(i, j0 + r - 1, rp[r], rn[r]);
-            }
+            for (int r = 0; r < TJ + 2; ++r) load_raw(i, j0 + min(r, nrows + 1) - 1, rp[r], rn[r]);
             rd = load_dens(i);
         };
         auto add_dep_rows = [&](S (&rp)[TJ + 2][V], S (&rn)[TJ + 2][V], double rd) {
             if (DEP) {
 #pragma unroll
-                for (int r = 0; r < TJ + 2; ++r)
-                    if (r <= nrows + 1) add_dep(row_dens(rd, r), rp[r], rn[r]);
+                for (int r = 0; r < TJ + 2; ++r) add_dep(row_dens(rd, r), rp[r], rn[r]);
             }
         };
         // store V values of (i, j) to the next arrays, lanes/values that own k
         auto store2 = [&](int i, int j, const S (&vp)[V], const S (&vn)[V]) {
-            const int64_t row = (int64_t)i * plane + (int64_t)j * nzb;
+            const int64_t row = (int64_t)i * plane + (int64_t)j * pitch;
             if constexpr (V == 2) {
                 if (k_out[0] && k_out[1]) {
                     *reinterpret_cast<f32x2 *>(PO + row + k0) = f32x2{vp[0], vp[1]};
@@ -337,19 +386,33 @@ __global__ void __launch_bounds__(512) sweep_kernel(SweepArgs a) {
             }
         };
 
-        // Four plane buffers in rotation: at plane i, buffer (i - i0) % 4 holds
-        // plane i-1, the next two hold planes i and i+1, and the fourth is
-        // being loaded with plane i+2. The march is unrolled by four so the
-        // rotation is a renaming of registers, not a copy.
-        S bp[4][TJ + 2][V], bn[4][TJ + 2][V];
-        double fd;  // deposition of the rows in flight, lane r = row r
-        load_rows(i0 - 1, bp[0], bn[0], fd);
-        add_dep_rows(bp[0], bn[0], fd);
+        // Plane buffers in rotation, NB = D + 3 of them for a lookahead of D
+        // planes: entering plane i, buffer (i - i0) % NB holds plane i-1 and
+        // the next one plane i (both ready: deposition added), and the D+1
+        // after them planes i+1 .. i+1+D, loaded in that order. A step issues
+        // the load of plane i+1+D, then adds the deposition to plane i+1 --
+        // issued D steps earlier, and with D planes of loads behind it, so the
+        // wait leaves those in flight -- then computes plane i. The march is
+        // unrolled by NB so the rotation is a renaming of registers, not a
+        // copy. Small tiles (few rows, so few registers per plane) look
+        // further ahead: on small grids there are too few waves to hide the
+        // memory latency of a plane by occupancy, so each wave has to keep
+        // several planes in flight.
+        constexpr int D = lookahead_planes<S, TJ, V>();
+        constexpr int NB = D + 3;
+        S bp[NB][TJ + 2][V], bn[NB][TJ + 2][V];
+        double fd[NB];  // deposition of each buffered plane's rows, lane r = row r
+        // every prologue load is issued before the first wait (planes past the
+        // chunk's last one, i1, repeat it and are never used)
+        static_for<D + 2>([&](auto Qc) {
+            constexpr int q = decltype(Qc)::value;
+            load_rows(min(i0 - 1 + q, i1), bp[q], bn[q], fd[q]);
+        });
+        static_for<2>([&](auto Qc) {
+            constexpr int q = decltype(Qc)::value;
+            add_dep_rows(bp[q], bn[q], fd[q]);
+        });
         if (!REFLECT && i0 == 1) carry_ring_rows(0, bp[0], bn[0], true);
-        load_rows(i0, bp[1], bn[1], fd);
-        add_dep_rows(bp[1], bn[1], fd);
-        load_rows(i0 + 1, bp[2], bn[2], fd);
-        add_dep_rows(bp[2], bn[2], fd);
 
         // per-lane sums of the scored voxels; lanes whose k is never scored are
         // masked once at the end (k_score is fixed for the whole march)
@@ -357,12 +420,16 @@ __global__ void __launch_bounds__(512) sweep_kernel(SweepArgs a) {
 #pragma unroll
         for (int v = 0; v < V; ++v) sr[v] = sp[v] = sn[v] = 0.0;
 
-        auto plane_step = [&](int i, auto Pc, auto Cc, auto Xc, auto Fc) {
-            constexpr int PB = decltype(Pc)::value, CB = decltype(Cc)::value;
-            constexpr int XB = decltype(Xc)::value, FB = decltype(Fc)::value;
-            const bool more = i + 1 < i1;
-            if (more) load_rows(i + 2, bp[FB], bn[FB], fd);
-            if (!REFLECT) {
+        auto plane_step = [&](int i, auto Oc) {
+            constexpr int O = decltype(Oc)::value;  // (i - i0) % NB
+            constexpr int PB = O % NB, CB = (O + 1) % NB, XB = (O + 2) % NB;
+            constexpr int LB = (O + 2 + D) % NB;  // plane i+1+D: loaded now
+            // every step of a group runs; one past the chunk (`live` false)
+            // loads a repeat of plane i1, computes, and stores nothing
+            const bool live = i < i1;
+            load_rows(min(i + 1 + D, i1), bp[LB], bn[LB], fd[LB]);
+            add_dep_rows(bp[XB], bn[XB], fd[XB]);
+            if (!REFLECT && live) {
                 carry_ring_rows(i, bp[CB], bn[CB], false);
                 if (i == no_xy - 2) carry_ring_rows(no_xy - 1, bp[XB], bn[XB], true);
             }
@@ -388,7 +455,7 @@ __global__ void __launch_bounds__(512) sweep_kernel(SweepArgs a) {
                     n_km[0] = __shfl_up(cn[r + 1][0], 1, kWave);
                     n_kp[0] = __shfl_down(cn[r + 1][0], 1, kWave);
                 }
-                if (r < nrows) {
+                if (live && r < nrows) {
                     const int j = j0 + r;
                     const double dj = (double)j - a.mid;
                     const bool inside = di2 + dj * dj < a.rsq;  // wave-uniform
@@ -440,17 +507,9 @@ __global__ void __launch_bounds__(512) sweep_kernel(SweepArgs a) {
                     store2(i, j, po, no);
                 }
             }
-            if (more) add_dep_rows(bp[FB], bn[FB], fd);
         };
-        using I0 = std::integral_constant<int, 0>;
-        using I1 = std::integral_constant<int, 1>;
-        using I2 = std::integral_constant<int, 2>;
-        using I3 = std::integral_constant<int, 3>;
-        for (int i = i0; i < i1; i += 4) {
-            plane_step(i, I0{}, I1{}, I2{}, I3{});
-            if (i + 1 < i1) plane_step(i + 1, I1{}, I2{}, I3{}, I0{});
-            if (i + 2 < i1) plane_step(i + 2, I2{}, I3{}, I0{}, I1{});
-            if (i + 3 < i1) plane_step(i + 3, I3{}, I0{}, I1{}, I2{});
+        for (int i = i0; i < i1; i += NB) {
+            static_for<NB>([&](auto Oc) { plane_step(i + decltype(Oc)::value, Oc); });
         }
 #pragma unroll
         for (int v = 0; v < V; ++v) {
@@ -461,10 +520,62 @@ __global__ void __launch_bounds__(512) sweep_kernel(SweepArgs a) {
     }
     block_sum3(acc_r, acc_p, acc_n);
     if (threadIdx.x == 0) {
-        a.partials[3 * (int64_t)blockIdx.x + 0] = acc_r;
-        a.partials[3 * (int64_t)blockIdx.x + 1] = acc_p;
-        a.partials[3 * (int64_t)blockIdx.x + 2] = acc_n;
+        a.partials[3 * (int64_t)logical + 0] = acc_r;
+        a.partials[3 * (int64_t)logical + 1] = acc_p;
+        a.partials[3 * (int64_t)logical + 2] = acc_n;
     }
+}
+
+// Fold the previous step's partials into its record (a block of its own, so
+// the fold's loads and its reduction run beside the march instead of ahead of
+// one of the marching blocks).
+__device__ __forceinline__ void fold_previous(const SweepArgs &a) {
+    double out[4];
+    block_final4(a.prev_part, a.prev_n, a.prev_dep_part, a.prev_n_dep, out);
+    if (threadIdx.x == 0)
+        for (int v = 0; v < 4; ++v) a.prev_out4[v] = out[v];
+}
+
+// One configuration, one step. Block 0 folds the previous step when one is
+// pending (prev_out4 set), the other blocks sweep. Shifting the sweep blocks
+// by one keeps each XCD's run of logical tiles on a single XCD.
+template <typename S, typename A, int TJ, bool DEP, int V, bool REFLECT>
+__global__ void __launch_bounds__(512) sweep_kernel(SweepArgs a) {
+    const int nf = a.prev_out4 ? 1 : 0;
+    if ((int)blockIdx.x < nf) {
+        fold_previous(a);
+        return;
+    }
+    sweep_body<S, A, TJ, V, REFLECT>(a, xcd_logical_block(blockIdx.x - nf, gridDim.x - nf), DEP);
+}
+
+// Many configurations, one step each, in one launch. Blocks [0, n_cfg) fold
+// configuration b's previous step (if pending); sweep block g (after the XCD
+// remap) belongs to configuration c with offs[c] <= g < offs[c+1] and runs it
+// with that configuration's arguments descs[c] -- its arrays, weights,
+// tiling and this step's deposition (dens null when the step deposits
+// nothing). Configurations are ordered so the active ones are a prefix; the
+// grid covers exactly those.
+template <typename S, typename A, int TJ, int V, bool REFLECT>
+__global__ void __launch_bounds__(512) batch_sweep_kernel(const SweepArgs *__restrict__ descs,
+                                                          const int *__restrict__ offs, int n_cfg) {
+    if ((int)blockIdx.x < n_cfg) {
+        const SweepArgs a = descs[blockIdx.x];
+        if (a.prev_out4) fold_previous(a);
+        return;
+    }
+    const int g = xcd_logical_block(blockIdx.x - n_cfg, gridDim.x - n_cfg);
+    // c = number of offs[1..n_cfg-1] <= g: one load per lane instead of a
+    // chain of dependent scalar loads
+    const int lane = threadIdx.x & 63;
+    int c = 0;
+    for (int base = 1; base < n_cfg; base += kWave) {
+        const int q = base + lane;
+        c += __popcll(__ballot(q < n_cfg && offs[q] <= g));
+    }
+    c = __builtin_amdgcn_readfirstlane(c);
+    const SweepArgs a = descs[c];
+    sweep_body<S, A, TJ, V, REFLECT>(a, g - offs[c], a.dens != nullptr);
 }
 
 struct GatherArgs {
@@ -480,7 +591,7 @@ struct GatherArgs {
 };
 
 constexpr int kGT = 16;  // gather tile edge
-constexpr int kGB = 32;  // tracks whose factors are evaluated per LDS batch
+constexpr int kGB = 128;  // tracks whose factors are evaluated per LDS batch
 
 __device__ __forceinline__ void track_box(double x, double y, double c, double nx, int &li, int &hi, int &lj,
                                           int &hj) {
@@ -497,14 +608,17 @@ __device__ __forceinline__ void track_box(double x, double y, double c, double n
 // row accumulation (solver_numba_parallel.py:183-219), so no atomics and a
 // deterministic result. The block first compacts, in order, the tracks whose
 // box meets the tile (ballot + prefix count), then evaluates each listed
-// track's 16 row and 16 column factors once into LDS, 32 tracks at a time.
+// track's 16 row and 16 column factors once into LDS, 128 tracks at a time
+// (few barriers, and sixteen independent exp evaluations per thread). A
+// factor whose row or column lies outside the track's box is stored as zero,
+// so the sum needs no box test: a skipped term and an added +0.0 leave the
+// (non-negative) sum bit-identical.
 // Output: dens = sum * gaussian_factor, and per block the scored injected
 // charge sum(dens * no_z) over interior columns inside the scoring disc
 // (solver_numba_parallel.py:250-277). Steps without tracks are skipped.
 __global__ void __launch_bounds__(256) gather_kernel(GatherArgs a) {
-    __shared__ int s_list[256];
+    __shared__ double s_x[256], s_y[256];  // centres of this chunk's tracks that reach the tile
     __shared__ int s_wcnt[4];
-    __shared__ int s_box[kGB][4];
     __shared__ double s_g[kGB][2 * kGT];
     const int64_t step = a.first_step + blockIdx.y;
     const int64_t t_begin = a.offsets[step];
@@ -523,9 +637,10 @@ __global__ void __launch_bounds__(256) gather_kernel(GatherArgs a) {
     for (int64_t base = 0; base < m; base += 256) {
         const int64_t t = base + tid;
         bool hit = false;
+        const double tx_c = xs[min(t, m - 1)], ty_c = ys[min(t, m - 1)];
         if (t < m) {
             int li, hi, lj, hj;
-            track_box(xs[t], ys[t], a.cutoff, nx, li, hi, lj, hj);
+            track_box(tx_c, ty_c, a.cutoff, nx, li, hi, lj, hj);
             hit = li < ti0 + kGT && hi > ti0 && lj < tj0 + kGT && hj > tj0;
         }
         const unsigned long long bal = __ballot(hit);
@@ -537,33 +652,28 @@ __global__ void __launch_bounds__(256) gather_kernel(GatherArgs a) {
             if (w < wave) off += s_wcnt[w];
             nlist += s_wcnt[w];
         }
-        if (hit) s_list[off + before] = tid;
+        // the list keeps the tracks' arrival order, so every voxel still adds
+        // them in ascending order
+        if (hit) {
+            s_x[off + before] = tx_c;
+            s_y[off + before] = ty_c;
+        }
         __syncthreads();
         for (int sb = 0; sb < nlist; sb += kGB) {
-            // 32 tracks x 32 factors: four per thread
-            for (int e = tid; e < kGB * 2 * kGT; e += 256) {
+            // 128 tracks x 32 factors: sixteen per thread
+            const int nq = min(kGB, nlist - sb);
+            for (int e = tid; e < nq * 2 * kGT; e += 256) {
                 const int q = e / (2 * kGT), f = e % (2 * kGT);
-                if (sb + q < nlist) {
-                    const int64_t tt = base + s_list[sb + q];
-                    const double x = xs[tt], y = ys[tt];
-                    const double d = f < kGT ? (double)(ti0 + f) - x : (double)(tj0 + f - kGT) - y;
-                    s_g[q][f] = exp(-(d * d) * a.h2b2);
-                    if (f == 0) {
-                        int li, hi, lj, hj;
-                        track_box(x, y, a.cutoff, nx, li, hi, lj, hj);
-                        s_box[q][0] = li;
-                        s_box[q][1] = hi;
-                        s_box[q][2] = lj;
-                        s_box[q][3] = hj;
-                    }
-                }
+                const double x = s_x[sb + q], y = s_y[sb + q];
+                int li, hi, lj, hj;
+                track_box(x, y, a.cutoff, nx, li, hi, lj, hj);
+                const int cell = f < kGT ? ti0 + f : tj0 + f - kGT;
+                const bool in_box = f < kGT ? (cell >= li && cell < hi) : (cell >= lj && cell < hj);
+                const double d = f < kGT ? (double)cell - x : (double)cell - y;
+                s_g[q][f] = in_box ? exp(-(d * d) * a.h2b2) : 0.0;
             }
             __syncthreads();
-            const int nq = min(kGB, nlist - sb);
-            for (int q2 = 0; q2 < nq; ++q2) {
-                if (i >= s_box[q2][0] && i < s_box[q2][1] && j >= s_box[q2][2] && j < s_box[q2][3])
-                    acc += s_g[q2][ty] * s_g[q2][kGT + tx];
-            }
+            for (int q2 = 0; q2 < nq; ++q2) acc += s_g[q2][ty] * s_g[q2][kGT + tx];
             __syncthreads();
         }
     }
@@ -582,7 +692,7 @@ __global__ void __launch_bounds__(256) gather_kernel(GatherArgs a) {
 // Explicit deposition into the current arrays (kernel-level API): add the
 // step's 2D density to every gap layer (solver_numba_parallel.py:253-262).
 template <typename S>
-__global__ void __launch_bounds__(256) broadcast_kernel(S *pos, S *neg, const double *dens, int no_xy, int nzb,
+__global__ void __launch_bounds__(256) broadcast_kernel(S *pos, S *neg, const double *dens, int no_xy, int pitch,
                                                         int nze, int no_z) {
     const int64_t n = (int64_t)no_xy * no_xy * no_z;
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
@@ -590,7 +700,7 @@ __global__ void __launch_bounds__(256) broadcast_kernel(S *pos, S *neg, const do
         const int64_t col = e / no_z;
         const double d = dens[col];
         if (d == 0.0) continue;
-        const int64_t idx = col * nzb + nze + kz;
+        const int64_t idx = col * pitch + nze + kz;
         pos[idx] = (S)((double)pos[idx] + d);
         neg[idx] = (S)((double)neg[idx] + d);
     }
@@ -608,29 +718,30 @@ __global__ void __launch_bounds__(256) reduce_kernel(const double *sweep_part, i
 }
 
 // Reflecting wall, for read-back only: write the lateral ring of an array
-// pair as the copy of the nearest interior column (state.py:113-154).
+// pair as the copy of the nearest interior column (state.py:113-154); whole
+// rows are copied, row padding (zero) included.
 template <typename S>
-__global__ void __launch_bounds__(256) ring_mirror_kernel(S *pos, S *neg, int no_xy, int nzb) {
-    const int64_t face = (int64_t)no_xy * nzb;        // one i plane
-    const int64_t side = (int64_t)(no_xy - 2) * nzb;  // j = 0 or no_xy-1, i = 1..no_xy-2
+__global__ void __launch_bounds__(256) ring_mirror_kernel(S *pos, S *neg, int no_xy, int pitch) {
+    const int64_t face = (int64_t)no_xy * pitch;        // one i plane
+    const int64_t side = (int64_t)(no_xy - 2) * pitch;  // j = 0 or no_xy-1, i = 1..no_xy-2
     const int64_t n = 2 * face + 2 * side;
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
         int i, j, k;
         if (e < 2 * face) {
             i = e < face ? 0 : no_xy - 1;
             const int64_t r = e % face;
-            j = (int)(r / nzb);
-            k = (int)(r % nzb);
+            j = (int)(r / pitch);
+            k = (int)(r % pitch);
         } else {
             const int64_t r = e - 2 * face;
             j = r < side ? 0 : no_xy - 1;
             const int64_t q = r % side;
-            i = 1 + (int)(q / nzb);
-            k = (int)(q % nzb);
+            i = 1 + (int)(q / pitch);
+            k = (int)(q % pitch);
         }
         const int ci = min(max(i, 1), no_xy - 2), cj = min(max(j, 1), no_xy - 2);
-        const int64_t dst = ((int64_t)i * no_xy + j) * nzb + k;
-        const int64_t src = ((int64_t)ci * no_xy + cj) * nzb + k;
+        const int64_t dst = ((int64_t)i * no_xy + j) * pitch + k;
+        const int64_t src = ((int64_t)ci * no_xy + cj) * pitch + k;
         pos[dst] = pos[src];
         neg[dst] = neg[src];
     }

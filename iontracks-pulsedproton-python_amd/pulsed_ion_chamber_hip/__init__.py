@@ -6,11 +6,14 @@ package's ``run_simulation_numba`` / ``run_simulation_numba_parallel`` /
 hand-written gfx950 HIP kernels behind a ctypes C ABI (include/pic_hip.h).
 ``SimulationConfig`` restates the reference's input contract so the backend
 runs without the reference installed; the reference's own config object works
-too.
+too. ``run_batch_hip(configs)`` runs a whole parameter sweep (uneven grids
+and step counts) in one device stream.
 """
 
+from .batch import HipBatch, run_batch_hip
 from .config import SimulationConfig
 from .solver_hip import HipSimulation, run_simulation_hip, warmup_hip
 from .state import Result
 
-__all__ = ["SimulationConfig", "Result", "run_simulation_hip", "warmup_hip", "HipSimulation"]
+__all__ = ["SimulationConfig", "Result", "run_simulation_hip", "warmup_hip", "HipSimulation", "run_batch_hip",
+           "HipBatch"]

@@ -48,6 +48,14 @@ EXPORTED = (
     "pic_set_tiling",
     "pic_enable_timing",
     "pic_get_timing",
+    "pic_batch_create",
+    "pic_batch_destroy",
+    "pic_batch_set_tiling",
+    "pic_batch_get_tiling",
+    "pic_batch_run",
+    "pic_batch_synchronize",
+    "pic_batch_enable_timing",
+    "pic_batch_get_timing",
 )
 
 
@@ -121,6 +129,14 @@ def open_library():
         "pic_set_tiling": [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int],
         "pic_enable_timing": [_VP, ctypes.c_int],
         "pic_get_timing": [_VP, _DP, _IP, _DP, _IP, _DP],
+        "pic_batch_create": [ctypes.POINTER(_VP), ctypes.c_int, ctypes.POINTER(_VP)],
+        "pic_batch_destroy": [_VP],
+        "pic_batch_set_tiling": [_VP, ctypes.c_int, ctypes.c_int],
+        "pic_batch_get_tiling": [_VP] + [ctypes.POINTER(ctypes.c_int)] * 4,
+        "pic_batch_run": [_VP, ctypes.c_int64, ctypes.c_int64],
+        "pic_batch_synchronize": [_VP],
+        "pic_batch_enable_timing": [_VP, ctypes.c_int],
+        "pic_batch_get_timing": [_VP, _DP, _IP, _DP, _IP],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

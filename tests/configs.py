@@ -52,3 +52,25 @@ def readme_config(grid_um=10.0, radius_cm=0.008, **overrides):
               memory_budget_fraction=None)
     kw.update(overrides)
     return SimulationConfig(**kw)
+
+
+# Saturation curve: 8 log-spaced voltages x 4 dose rates, the README beam and
+# chamber otherwise; each point gets a grid from the divisors of the 2 mm gap
+# and a sampled radius drawn from default_rng(2026), so the 32 grids and step
+# counts are uneven.
+CURVE_VOLTAGES_V = tuple(round(50.0 * 10.0 ** (k / 7.0), 2) for k in range(8))
+CURVE_DOSE_RATES_GY_S = (2.0, 8.91, 20.0, 40.0)
+CURVE_GRIDS_UM = (10.0, 12.5, 16.0, 20.0)
+
+
+def saturation_curve_configs(seed=2026, voltages=CURVE_VOLTAGES_V, dose_rates=CURVE_DOSE_RATES_GY_S):
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    out = []
+    for voltage in voltages:
+        for dose in dose_rates:
+            grid = CURVE_GRIDS_UM[int(rng.integers(len(CURVE_GRIDS_UM)))]
+            radius = float(rng.uniform(0.005, 0.02))
+            out.append(readme_config(grid, radius, voltage_V=voltage, dose_rate_Gy_s=dose))
+    return out

@@ -99,13 +99,19 @@ def test_deposit_matches_the_batched_oracle_and_is_deterministic():
     assert outs[0][0] == pytest.approx(ref, rel=1e-13)
 
 
+# 40 um: 8 x 8 x 56 voxels; 16 um: 11 x 11 x 131, an odd number of k voxels
+# (the device pads FP32 rows to an even length)
+GRIDS_UM = [40.0, 16.0]
+
+
+@pytest.mark.parametrize("grid_um", GRIDS_UM)
 @pytest.mark.parametrize("wall", ["absorbing", "reflecting"])
 @pytest.mark.parametrize("species", ["one", "two"])
-def test_one_fp64_sweep_is_bit_identical_to_the_oracle(wall, species):
+def test_one_fp64_sweep_is_bit_identical_to_the_oracle(wall, species, grid_um):
     """Same input, same operations in the same order, no fused multiply-adds:
     the device sweep must reproduce the oracle's interior exactly."""
     extra = TWO_SPECIES if species == "two" else {}
-    config = SimulationConfig(**CC, **extra, lateral_boundary=wall)
+    config = SimulationConfig(**{**CC, "grid_size_um": grid_um}, **extra, lateral_boundary=wall)
     rng = np.random.default_rng(17)
     shape = (config.no_xy, config.no_xy, config.no_z_with_buffer)
     pos = rng.random(shape) * 1e7
@@ -141,14 +147,18 @@ def test_one_fp64_sweep_is_bit_identical_to_the_oracle(wall, species):
             np.testing.assert_array_equal(a[:, -1], m[:, -1])
 
 
-def test_one_fp32_sweep_stays_within_fp32_rounding_of_the_oracle():
-    config = SimulationConfig(**CC, **TWO_SPECIES)
+@pytest.mark.parametrize("grid_um", GRIDS_UM)
+@pytest.mark.parametrize("wall", ["absorbing", "reflecting"])
+def test_one_fp32_sweep_stays_within_fp32_rounding_of_the_oracle(wall, grid_um):
+    config = SimulationConfig(**{**CC, "grid_size_um": grid_um}, **TWO_SPECIES, lateral_boundary=wall)
     rng = np.random.default_rng(19)
     shape = (config.no_xy, config.no_xy, config.no_z_with_buffer)
     pos = (rng.random(shape) * 1e7).astype(np.float32).astype(np.float64)
     neg = (rng.random(shape) * 1e7).astype(np.float32).astype(np.float64)
     pos[:, :, [0, -1]] = 0.0
     neg[:, :, [0, -1]] = 0.0
+    serial.apply_lateral_boundary(pos, wall)
+    serial.apply_lateral_boundary(neg, wall)
     ref_p, ref_n, ref3 = batched.lw_step(config, pos, neg)
     with HipSimulation(config, "fp32") as sim:
         sim.write_fields(pos, neg)
@@ -159,3 +169,23 @@ def test_one_fp32_sweep_stays_within_fp32_rounding_of_the_oracle():
     assert np.abs(gp[c] - ref_p[c]).max() <= 4e-7 * scale
     assert np.abs(gn[c] - ref_n[c]).max() <= 4e-7 * scale
     np.testing.assert_allclose(got3, ref3, rtol=1e-6)
+
+
+@pytest.mark.parametrize("grid_um", GRIDS_UM)
+def test_fp32_one_and_two_values_per_lane_give_the_same_fields(grid_um):
+    """The two FP32 layouts (4-byte and 8-byte lanes over padded rows) run the
+    same voxel update: a whole run must agree bit for bit."""
+    from pulsed_ion_chamber_hip.pulses import draw_tracks
+
+    config = SimulationConfig(**{**CC, "grid_size_um": grid_um}, **TWO_SPECIES)
+    draw = draw_tracks(config)
+    out = []
+    for v in (1, 2):
+        with HipSimulation(config, "fp32") as sim:
+            sim.set_tiling(values_per_lane=v)
+            sim.upload_tracks(*draw)
+            sim.run_steps(0, config.total_time_steps)
+            out.append((sim.read_per_step(), *sim.read_fields()))
+    np.testing.assert_array_equal(out[0][1], out[1][1])
+    np.testing.assert_array_equal(out[0][2], out[1][2])
+    np.testing.assert_allclose(out[0][0], out[1][0], rtol=1e-12, atol=1e-300)
