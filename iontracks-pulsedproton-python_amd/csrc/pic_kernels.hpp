@@ -481,8 +481,12 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &a, const int logical
                             // FP32 fields: flux form x + sum_w w (x_w - x). The weights
                             // sum to one, so this is the same update, and written as
                             // fluxes it moves charge between neighbours without
-                            // creating any, whatever the rounding of the FP32 weights;
-                            // the recombination product stays FP64.
+                            // creating any, whatever the rounding of the FP32 weights.
+                            // The recombination product and the final x + flux - rec
+                            // are FP64 (x + flux is exact there), so the stored value
+                            // is rounded to FP32 once: on the strongest-recombination
+                            // point of a saturation curve (50 V, 40 Gy/s) that keeps
+                            // |dk_s| at 2.3e-7 where two FP32 roundings reach 9.5e-7.
                             const float p = cp[r + 1][v], n = cn[r + 1][v];
                             const float tp = f_pzm * (p_km[v] - p) + f_pzp * (p_kp[v] - p) +
                                              f_plat * (((cp[r][v] - p) + (cp[r + 2][v] - p)) +
@@ -491,9 +495,8 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &a, const int logical
                                              f_nlat * (((cn[r][v] - n) + (cn[r + 2][v] - n)) +
                                                        ((im_n - n) + (ip_n - n)));
                             rec[v] = a.alpha_dt * (double)p * (double)n;
-                            const float rf = (float)rec[v];
-                            po[v] = (p + tp) - rf;
-                            no[v] = (n + tn) - rf;
+                            po[v] = (float)(((double)p + (double)tp) - rec[v]);
+                            no[v] = (float)(((double)n + (double)tn) - rec[v]);
                         }
                     }
                     if (inside) {
