@@ -43,6 +43,7 @@ extern "C" {
 
 typedef struct pic_sim pic_sim;
 typedef struct pic_batch pic_batch;
+typedef struct pic_adjoint pic_adjoint;
 
 /* Everything the loop reads from SimulationConfig (config.py:180-401,
  * scheme_coefficients config.py:466-482, alpha_dt solver_numba.py:273-274). */
@@ -143,8 +144,10 @@ int pic_get_timing(pic_sim *sim, double *sweep_ms, int64_t *sweep_launches, doub
  * the handles (destroy it first). */
 int pic_batch_create(pic_sim *const *sims, int n, pic_batch **out);
 int pic_batch_destroy(pic_batch *batch);
-/* Common sweep tiling (rows per tile 1, 2 or 4 -- FP64: 1 or 2 -- and planes
- * per block chunk; 0 = automatic) and the tiling in use. */
+/* Sweep tiling of every step (rows per tile 1, 2 or 4 -- FP64: 1 or 2 -- and
+ * planes per block chunk), or 0, 0 for the automatic choice, made per step
+ * from the configurations still running; get_tiling reports the preferred
+ * (first) tiling, the value layout and the waves per block. */
 int pic_batch_set_tiling(pic_batch *batch, int rows_per_tile, int planes_per_chunk);
 int pic_batch_get_tiling(pic_batch *batch, int *rows_per_tile, int *planes_per_chunk, int *values_per_lane,
                          int *windows_per_block);
@@ -159,6 +162,27 @@ int pic_batch_synchronize(pic_batch *batch);
 int pic_batch_enable_timing(pic_batch *batch, int enable);
 int pic_batch_get_timing(pic_batch *batch, double *sweep_ms, int64_t *sweep_launches, double *gather_ms,
                          int64_t *gather_launches);
+
+/* Reverse-mode derivative of one FP64 handle's run (no counterpart in the
+ * reference, which has no derivatives; it differentiates the loop the
+ * reference runs, solver_numba.py:237-370). The forward pass keeps a
+ * checkpoint of the state every checkpoint_every steps (capped at the
+ * handle's gather batch); the reverse pass recomputes each segment from its
+ * checkpoint and runs the adjoint steps backwards through it.
+ * pic_adjoint_run resets the handle and runs the uploaded schedule:
+ *   sums  = (sum of injected, sum of recombined) charge over the run,
+ *   d_rec = d(sum recombined) / d(p_lat, p_zm, p_zp, p_cen, n_lat, n_zm, n_zp,
+ *           n_cen, alpha_dt, gaussian_factor, h2_over_b2) -- the pic_params
+ *           fields, the deposition box held fixed,
+ *   d_inj = d(sum injected) / d(gaussian_factor, h2_over_b2).
+ * k_s = sum_inj / (sum_inj - sum_rec) and its derivatives follow on the host. */
+int pic_adjoint_create(pic_sim *sim, int64_t checkpoint_every, pic_adjoint **out);
+int pic_adjoint_destroy(pic_adjoint *adj);
+int pic_adjoint_run(pic_adjoint *adj, double sums[2], double d_rec[11], double d_inj[2]);
+/* Segment length in use, device bytes held, and the last run's forward and
+ * reverse times (HIP events on the handle's stream). */
+int pic_adjoint_get_info(pic_adjoint *adj, int64_t *checkpoint_every, size_t *device_bytes, double *forward_ms,
+                         double *backward_ms);
 
 #ifdef __cplusplus
 }

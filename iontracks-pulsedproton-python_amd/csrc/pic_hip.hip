@@ -19,6 +19,7 @@
 
 #include "pic_hip.h"
 #include "pic_kernels.hpp"
+#include "pic_adjoint.hpp"
 
 namespace {
 
@@ -98,12 +99,12 @@ struct pic_batch {
     std::vector<pic_sim *> sims;  // caller's order
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
-    int prec = PIC_FP64, v = 1, tj = 0, wpb = 1;
+    int prec = PIC_FP64, v = 1, wpb = 1;
     bool reflect = false;
     int force_tj = 0, force_chunk = 0;
-    int64_t waves_wanted = 0;
-    std::vector<int> chunks;                      // candidate planes per block, longest first
-    std::vector<std::vector<SweepShape>> shapes;  // [chunk index][sims[c]]
+    int64_t waves_min = 0;
+    std::vector<std::pair<int, int>> cands;       // candidate (rows per tile, planes per block), preferred first
+    std::vector<std::vector<SweepShape>> shapes;  // [candidate][sims[c]]
     // Per-segment argument tables (a segment is one gather batch of steps),
     // double-buffered so the host builds segment k+1 while the device runs
     // segment k: pinned host staging, device copy, and the event that marks
@@ -117,6 +118,27 @@ struct pic_batch {
     std::vector<hipEvent_t> ev_pool;  // pairs around each launch: kind 0 sweep, 1 gather
     std::vector<int> ev_kind;
     size_t ev_used = 0;
+};
+
+// Reverse-mode derivative of one handle's run: checkpoints of the state
+// every K steps during the forward pass; per segment (last first) the states
+// are recomputed from the checkpoint into a store of K+1 array pairs, then
+// the adjoint steps run backwards through them.
+struct pic_adjoint {
+    pic_sim *s = nullptr;
+    int64_t K = 0, nseg = 0, T = 0;
+    double *ckpt = nullptr;   // [nseg][2][n_vox]
+    double *store = nullptr;  // [K + 1][2][n_vox]
+    double *mu = nullptr;     // [2 buffers][2 species][n_vox]
+    double *dgf = nullptr, *dh = nullptr;  // [K][no_xy^2]
+    double *dpart = nullptr;               // [K][gather tiles][2]
+    double *apart = nullptr;               // [2][adjoint blocks][kAdjStride]
+    double *grad_rec = nullptr;            // [T][kAdjNV]
+    double *dinj_rec = nullptr;            // [T][2]
+    double *scratch = nullptr;             // sweep records of the recompute (discarded)
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    float forward_ms = 0, backward_ms = 0;
+    size_t bytes = 0;
 };
 
 namespace {
@@ -447,54 +469,38 @@ size_t partial_cap_bytes(const pic_params *p) {
     return (size_t)cap * 3 * sizeof(double);
 }
 
-// Common tiling of a batch: one block size (wpb waves) and one kernel
-// instance (rows per tile, values per lane) for every configuration, chosen
-// like sweep_shape but for the waves of all configurations together. The
-// planes per block may change from step to step: each step takes the longest
-// chunk that still gives the configurations running at that step enough
-// waves, so the tail of a curve, where one or two grids are left, marches in
-// short chunks.
+// Tiling of a batch: one block size (wpb waves) and one value layout for
+// every configuration; the tile rows and planes per block are chosen per step
+// from a short candidate list -- the first candidate whose blocks give the
+// configurations running at that step at least waves_min waves. Measured on
+// MI355X over a 32-point saturation curve (16^2..47^2 x 106..206 grids):
+// 2-row tiles of 4 planes are fastest while eight or more grids run, 1-row
+// tiles of 4 planes once a few are left; taller tiles lose to the register
+// cost of the multi-configuration kernel, longer chunks to the march length.
 int batch_tiling(pic_batch *b) {
     int v = 2, wpb = 8;
     for (pic_sim *s : b->sims) v = std::min(v, values_per_lane(s, false));
     for (pic_sim *s : b->sims)
         wpb = std::min(wpb, (int)ceil_div(s->p.nzb - 2, pic::kWave * v - 2));
     const int cus = sm_count(b->sims[0]->device);
-    const int64_t waves_wanted = (int64_t)cus * 16;
-    auto waves = [&](int tj, int chunk) {
-        int64_t w = 0;
-        for (pic_sim *s : b->sims) w += fixed_shape(s, v, tj, chunk, wpb).blocks() * wpb;
-        return w;
-    };
-    int tj = b->force_tj;
-    const int max_tj = b->prec == PIC_FP64 ? 2 : 4;
-    if (tj <= 0) {
-        tj = 1;
-        for (int c = max_tj; c >= 1; c /= 2)
-            if (waves(c, b->force_chunk > 0 ? b->force_chunk : 16) >= waves_wanted) {
-                tj = c;
-                break;
-            }
-    }
-    std::vector<int> chunks;
-    if (b->force_chunk > 0)
-        chunks = {b->force_chunk};
+    std::vector<std::pair<int, int>> cands;
+    if (b->force_tj > 0 || b->force_chunk > 0)
+        cands = {{b->force_tj > 0 ? b->force_tj : 1, b->force_chunk > 0 ? b->force_chunk : 4}};
     else
-        chunks = {16, 8, 4};
-    std::vector<std::vector<SweepShape>> shapes(chunks.size());
-    for (size_t ci = 0; ci < chunks.size(); ++ci)
-        for (pic_sim *s : b->sims) shapes[ci].push_back(fixed_shape(s, v, tj, chunks[ci], wpb));
+        cands = {{2, 4}, {1, 4}};
+    std::vector<std::vector<SweepShape>> shapes(cands.size());
+    for (size_t ci = 0; ci < cands.size(); ++ci)
+        for (pic_sim *s : b->sims) shapes[ci].push_back(fixed_shape(s, v, cands[ci].first, cands[ci].second, wpb));
     for (size_t c = 0; c < b->sims.size(); ++c) {
         int64_t most = 0;
-        for (size_t ci = 0; ci < chunks.size(); ++ci) most = std::max(most, shapes[ci][c].blocks());
+        for (size_t ci = 0; ci < cands.size(); ++ci) most = std::max(most, shapes[ci][c].blocks());
         const int rc = ensure_partials(b->sims[c], most);
         if (rc) return rc;
     }
     b->v = v;
     b->wpb = wpb;
-    b->tj = tj;
-    b->waves_wanted = waves_wanted;
-    b->chunks = chunks;
+    b->waves_min = (int64_t)cus * 6;
+    b->cands = cands;
     b->shapes = shapes;
     return PIC_OK;
 }
@@ -528,6 +534,123 @@ int ensure_batch_slots(pic_batch *b, int64_t steps) {
     }
     b->slot_steps = steps;
     b->slot_cfgs = n;
+    return PIC_OK;
+}
+
+// --- adjoint helpers -------------------------------------------------
+
+int64_t adjoint_blocks(const pic_sim *s) {
+    return ceil_div(s->p.nzb, pic::kWave) * ceil_div(s->p.no_xy, 4) * s->p.no_xy;
+}
+
+// one forward sweep of step t from store slot src to slot src+1, with the
+// step's deposition (dens slot of the sim) when it has tracks; the step's
+// record is not kept
+int adjoint_recompute_sweep(pic_adjoint *ad, int64_t t, int64_t src) {
+    pic_sim *s = ad->s;
+    const SweepShape sh = sweep_shape(s);
+    const int64_t xy = s->p.no_xy * s->p.no_xy;
+    const bool dep = s->offsets[t + 1] > s->offsets[t];
+    const int n_tiles = (int)gather_blocks(s);
+    const int64_t slot = t % (s->batch + 1);
+    s->pending = pic_sim::Pending{};
+    pic::SweepArgs a;
+    int rc = make_sweep_args(s, sh, 0, dep, s->dens + slot * xy, s->dep_part + slot * n_tiles, n_tiles, ad->scratch, a);
+    if (rc) return rc;
+    s->pending = pic_sim::Pending{};
+    double *in = ad->store + src * 2 * s->n_vox;
+    a.pos = in;
+    a.neg = in + s->n_vox;
+    a.pos_out = in + 2 * s->n_vox;
+    a.neg_out = in + 3 * s->n_vox;
+    a.prev_out4 = nullptr;
+    const dim3 grid((unsigned)sh.blocks()), block((unsigned)(64 * sh.wpb));
+    rc = launch_sweep_typed<double, double, 1>(a, sh.tj, grid, block, s->stream, dep, s->p.wall == PIC_WALL_REFLECTING);
+    if (rc) return rc;
+    HIP_LAUNCH_CHECK();
+    return PIC_OK;
+}
+
+int adjoint_deriv_gather(pic_adjoint *ad, int64_t first, int64_t count) {
+    pic_sim *s = ad->s;
+    pic::GatherArgs g{};
+    g.xs = s->xs;
+    g.ys = s->ys;
+    g.offsets = s->offsets_d;
+    g.first_step = first;
+    g.n_slots = s->batch + 1;
+    g.no_xy = (int)s->p.no_xy;
+    g.no_z = (int)s->p.no_z;
+    g.ntj = (int)ceil_div(s->p.no_xy, pic::kGT);
+    g.cutoff = s->p.cutoff_voxels;
+    g.h2b2 = s->p.h2_over_b2;
+    g.gf = s->p.gaussian_factor;
+    g.mid = s->p.mid_xy;
+    g.rsq = s->p.scoring_radius_sq;
+    g.dgf = ad->dgf;
+    g.dh = ad->dh;
+    g.dpart = ad->dpart;
+    const int tiles = (int)gather_blocks(s);
+    hipLaunchKernelGGL(pic::gather_deriv_kernel, dim3((unsigned)tiles, (unsigned)count), dim3(256), 0, s->stream, g);
+    HIP_LAUNCH_CHECK();
+    hipLaunchKernelGGL(pic::fold_rows_kernel<2>, dim3((unsigned)count), dim3(256), 0, s->stream, ad->dpart, tiles, 2,
+                       ad->dinj_rec + 2 * first);
+    HIP_LAUNCH_CHECK();
+    return PIC_OK;
+}
+
+struct AdjPending {
+    const double *part = nullptr;
+    int n = 0;
+    double *out = nullptr;
+};
+
+// adjoint step t: mu_t from mu_{t+1} (buffer `cur`) into buffer cur ^ 1
+int adjoint_step(pic_adjoint *ad, int64_t t, int64_t src, int cur, AdjPending &pend) {
+    pic_sim *s = ad->s;
+    const int64_t xy = s->p.no_xy * s->p.no_xy;
+    const bool dep = s->offsets[t + 1] > s->offsets[t];
+    const int64_t nblk = adjoint_blocks(s);
+    pic::AdjointArgs a{};
+    const double *in = ad->store + src * 2 * s->n_vox;
+    a.sp = in;
+    a.sn = in + s->n_vox;
+    a.dens = dep ? s->dens + (t % (s->batch + 1)) * xy : nullptr;
+    a.dgf = dep ? ad->dgf + (t % ad->K) * xy : nullptr;
+    a.dh = dep ? ad->dh + (t % ad->K) * xy : nullptr;
+    a.mp = ad->mu + (2 * cur + 0) * s->n_vox;
+    a.mn = ad->mu + (2 * cur + 1) * s->n_vox;
+    a.op = ad->mu + (2 * (cur ^ 1) + 0) * s->n_vox;
+    a.on = ad->mu + (2 * (cur ^ 1) + 1) * s->n_vox;
+    a.partials = ad->apart + (t % 2) * nblk * pic::kAdjStride;
+    a.prev_part = pend.part;
+    a.prev_n = pend.n;
+    a.prev_out = pend.out;
+    a.no_xy = (int)s->p.no_xy;
+    a.nzb = (int)s->p.nzb;
+    a.nze = (int)s->p.nze;
+    a.no_z = (int)s->p.no_z;
+    a.pitch = (int)s->pitch;
+    a.n_kb = (int)ceil_div(s->p.nzb, pic::kWave);
+    a.n_jb = (int)ceil_div(s->p.no_xy, 4);
+    a.mid = s->p.mid_xy;
+    a.rsq = s->p.scoring_radius_sq;
+    a.p_lat = s->p.p_lat;
+    a.p_zm = s->p.p_zm;
+    a.p_zp = s->p.p_zp;
+    a.p_cen = s->p.p_cen;
+    a.n_lat = s->p.n_lat;
+    a.n_zm = s->p.n_zm;
+    a.n_zp = s->p.n_zp;
+    a.n_cen = s->p.n_cen;
+    a.adt = s->p.alpha_dt;
+    const dim3 grid((unsigned)(nblk + (pend.out ? 1 : 0))), block(256);
+    if (s->p.wall == PIC_WALL_REFLECTING)
+        hipLaunchKernelGGL(pic::adjoint_kernel<true>, grid, block, 0, s->stream, a);
+    else
+        hipLaunchKernelGGL(pic::adjoint_kernel<false>, grid, block, 0, s->stream, a);
+    HIP_LAUNCH_CHECK();
+    pend = AdjPending{a.partials, (int)nblk, ad->grad_rec + pic::kAdjNV * t};
     return PIC_OK;
 }
 
@@ -971,8 +1094,8 @@ int pic_batch_set_tiling(pic_batch *b, int rows_per_tile, int planes_per_chunk) 
 int pic_batch_get_tiling(pic_batch *b, int *rows_per_tile, int *planes_per_chunk, int *values_per_lane,
                          int *windows_per_block) {
     if (!b) return fail(PIC_E_INVALID, "batch is NULL");
-    if (rows_per_tile) *rows_per_tile = b->tj;
-    if (planes_per_chunk) *planes_per_chunk = b->chunks.empty() ? 0 : b->chunks[0];
+    if (rows_per_tile) *rows_per_tile = b->cands.empty() ? 0 : b->cands[0].first;
+    if (planes_per_chunk) *planes_per_chunk = b->cands.empty() ? 0 : b->cands[0].second;
     if (values_per_lane) *values_per_lane = b->v;
     if (windows_per_block) *windows_per_block = b->wpb;
     return PIC_OK;
@@ -1003,7 +1126,7 @@ int pic_batch_run(pic_batch *b, int64_t first, int64_t count) {
     for (pic_sim *s : b->sims) s->pending = pic_sim::Pending{};  // earlier runs were flushed
     b->ev_used = 0;
     const dim3 block((unsigned)(64 * b->wpb));
-    std::vector<int> active, chunk_of;
+    std::vector<int> active, cand_of;
     int k = 0;
     // one segment per gather batch: gathers first, then the steps' sweeps
     for (int64_t s0 = first; s0 < first + count && rc == PIC_OK; s0 += gbatch, k ^= 1) {
@@ -1012,17 +1135,19 @@ int pic_batch_run(pic_batch *b, int64_t first, int64_t count) {
         pic::SweepArgs *descs = b->descs_h[k];
         int *offs = b->offs_h[k];
         active.assign(s1 - s0, 0);
+        cand_of.assign(s1 - s0, 0);
         // descriptors of every (step, configuration) of the segment, built in
         // step order so each handle's buffer swaps and pending steps advance
         for (int64_t t = s0; t < s1 && rc == PIC_OK; ++t) {
             int na = 0;
             while (na < n && t < b->sims[ord[na]]->n_steps) ++na;
-            size_t ci = 0;  // longest chunk that keeps the device busy
-            for (; ci + 1 < b->chunks.size(); ++ci) {
+            size_t ci = 0;  // first candidate tiling that keeps the device busy
+            for (; ci + 1 < b->cands.size(); ++ci) {
                 int64_t w = 0;
                 for (int q = 0; q < na; ++q) w += b->shapes[ci][ord[q]].blocks() * b->wpb;
-                if (w >= b->waves_wanted) break;
+                if (w >= b->waves_min) break;
             }
+            cand_of[t - s0] = (int)ci;
             int *o = offs + (t - s0) * (n + 1);
             o[0] = 0;
             for (int q = 0; q < na && rc == PIC_OK; ++q) {
@@ -1064,10 +1189,11 @@ int pic_batch_run(pic_batch *b, int64_t first, int64_t count) {
             const dim3 grid((unsigned)(o[na] + na));  // na fold blocks, then the sweep blocks
             const pic::SweepArgs *d = b->descs_d[k] + (t - s0) * n;
             const int *od = b->offs_d[k] + (t - s0) * (n + 1);
+            const int tj = b->cands[cand_of[t - s0]].first;
             record_batch(b, 0);
-            rc = b->prec == PIC_FP64 ? launch_batch_typed<double, double, 1>(d, od, na, b->tj, grid, block, b->stream, b->reflect)
-                 : b->v == 2         ? launch_batch_typed<float, float, 2>(d, od, na, b->tj, grid, block, b->stream, b->reflect)
-                                     : launch_batch_typed<float, float, 1>(d, od, na, b->tj, grid, block, b->stream, b->reflect);
+            rc = b->prec == PIC_FP64 ? launch_batch_typed<double, double, 1>(d, od, na, tj, grid, block, b->stream, b->reflect)
+                 : b->v == 2         ? launch_batch_typed<float, float, 2>(d, od, na, tj, grid, block, b->stream, b->reflect)
+                                     : launch_batch_typed<float, float, 1>(d, od, na, tj, grid, block, b->stream, b->reflect);
             if (!rc && hipGetLastError() != hipSuccess) rc = fail(PIC_E_HIP, "batched sweep launch failed");
             record_batch(b, 0);
         }
@@ -1109,6 +1235,160 @@ int pic_batch_get_timing(pic_batch *b, double *sweep_ms, int64_t *sweep_launches
 int pic_batch_synchronize(pic_batch *b) {
     if (!b) return fail(PIC_E_INVALID, "batch is NULL");
     HIP_TRY(hipStreamSynchronize(b->stream));
+    return PIC_OK;
+}
+
+int pic_adjoint_create(pic_sim *s, int64_t checkpoint_every, pic_adjoint **out) {
+    if (!out) return fail(PIC_E_INVALID, "out is NULL");
+    *out = nullptr;
+    if (!s) return fail(PIC_E_INVALID, "sim is NULL");
+    if (s->prec != PIC_FP64) return fail(PIC_E_INVALID, "the adjoint needs FP64 fields");
+    if (s->p.no_xy < 4) return fail(PIC_E_INVALID, "the adjoint needs no_xy >= 4");
+    if (s->n_steps <= 0) return fail(PIC_E_INVALID, "upload the tracks before creating the adjoint");
+    if (checkpoint_every < 1) return fail(PIC_E_INVALID, "checkpoint_every must be positive");
+    pic_adjoint *ad = new pic_adjoint();
+    ad->s = s;
+    // a segment's densities must stay in the handle's gather slots until its
+    // adjoint steps are done
+    ad->K = std::min<int64_t>(checkpoint_every, s->batch);
+    ad->T = s->n_steps;
+    ad->nseg = ceil_div(ad->T, ad->K);
+    const int64_t xy = s->p.no_xy * s->p.no_xy, tiles = gather_blocks(s), nblk = adjoint_blocks(s);
+    const size_t pair = 2 * (size_t)s->n_vox * 8;
+    const size_t sizes[] = {ad->nseg * pair, (size_t)(ad->K + 1) * pair, 2 * pair, (size_t)ad->K * xy * 8,
+                            (size_t)ad->K * xy * 8, (size_t)ad->K * tiles * 2 * 8,
+                            (size_t)2 * nblk * pic::kAdjStride * 8, (size_t)ad->T * pic::kAdjNV * 8,
+                            (size_t)ad->T * 2 * 8, 64};
+    double **ptrs[] = {&ad->ckpt, &ad->store, &ad->mu, &ad->dgf, &ad->dh, &ad->dpart,
+                       &ad->apart, &ad->grad_rec, &ad->dinj_rec, &ad->scratch};
+    size_t total = 0;
+    for (size_t b : sizes) total += b;
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+        delete ad;
+        return fail(PIC_E_HIP, "hipMemGetInfo failed");
+    }
+    if ((double)total > 0.9 * (double)fr) {
+        delete ad;
+        return fail(PIC_E_NOMEM, "the adjoint needs %.3f GiB (checkpoints every %lld steps), %.3f GiB free",
+                    total / 1073741824.0, (long long)ad->K, fr / 1073741824.0);
+    }
+    for (size_t q = 0; q < sizeof sizes / sizeof sizes[0]; ++q) {
+        if (hipMalloc(ptrs[q], sizes[q]) != hipSuccess) {
+            pic_adjoint_destroy(ad);
+            return fail(PIC_E_NOMEM, "hipMalloc for the adjoint failed");
+        }
+        if (hipMemset(*ptrs[q], 0, sizes[q]) != hipSuccess) {
+            pic_adjoint_destroy(ad);
+            return fail(PIC_E_HIP, "hipMemset for the adjoint failed");
+        }
+    }
+    for (hipEvent_t &e : ad->ev)
+        if (hipEventCreateWithFlags(&e, hipEventReleaseToDevice) != hipSuccess) {
+            pic_adjoint_destroy(ad);
+            return fail(PIC_E_HIP, "hipEventCreate failed");
+        }
+    ad->bytes = total;
+    *out = ad;
+    return PIC_OK;
+}
+
+int pic_adjoint_destroy(pic_adjoint *ad) {
+    if (!ad) return PIC_OK;
+    if (ad->s && ad->s->stream) hipStreamSynchronize(ad->s->stream);
+    double *bufs[] = {ad->ckpt, ad->store, ad->mu, ad->dgf, ad->dh, ad->dpart, ad->apart, ad->grad_rec,
+                      ad->dinj_rec, ad->scratch};
+    for (double *b : bufs)
+        if (b) hipFree(b);
+    for (hipEvent_t e : ad->ev)
+        if (e) hipEventDestroy(e);
+    delete ad;
+    return PIC_OK;
+}
+
+int pic_adjoint_run(pic_adjoint *ad, double sums[2], double d_rec[11], double d_inj[2]) {
+    if (!ad) return fail(PIC_E_INVALID, "adjoint is NULL");
+    pic_sim *s = ad->s;
+    if (s->n_steps != ad->T) return fail(PIC_E_INVALID, "the handle's tracks changed since the adjoint was created");
+    const int64_t T = ad->T, K = ad->K, nv = s->n_vox;
+    int rc = pic_reset(s);
+    if (rc) return rc;
+    // forward, a checkpoint at the start of every segment
+    HIP_TRY(hipEventRecord(ad->ev[0], s->stream));
+    for (int64_t k = 0; k < ad->nseg && rc == PIC_OK; ++k) {
+        double *c = ad->ckpt + k * 2 * nv;
+        HIP_TRY(hipMemcpyAsync(c, s->cur_p, nv * 8, hipMemcpyDeviceToDevice, s->stream));
+        HIP_TRY(hipMemcpyAsync(c + nv, s->cur_n, nv * 8, hipMemcpyDeviceToDevice, s->stream));
+        rc = pic_run_steps(s, k * K, std::min(K, T - k * K));
+    }
+    if (rc) return rc;
+    HIP_TRY(hipEventRecord(ad->ev[1], s->stream));
+    // reverse sweep
+    hipLaunchKernelGGL(pic::fill_kernel<double>, dim3(1024), dim3(256), 0, s->stream, ad->mu, 4 * nv);
+    HIP_LAUNCH_CHECK();
+    hipLaunchKernelGGL(pic::fill_kernel<double>, dim3(256), dim3(256), 0, s->stream, ad->dinj_rec, 2 * T);
+    HIP_LAUNCH_CHECK();
+    AdjPending pend;
+    int cur = 0;
+    for (int64_t k = ad->nseg - 1; k >= 0 && rc == PIC_OK; --k) {
+        const int64_t s0 = k * K, e = std::min(T, s0 + K);
+        HIP_TRY(hipMemcpyAsync(ad->store, ad->ckpt + k * 2 * nv, 2 * nv * 8, hipMemcpyDeviceToDevice, s->stream));
+        if (s->offsets[e] > s->offsets[s0]) {
+            rc = enqueue_gather(s, s->xs, s->ys, s->offsets_d, s0, e - s0);
+            if (!rc) rc = adjoint_deriv_gather(ad, s0, e - s0);
+        }
+        for (int64_t t = s0; t + 1 < e && rc == PIC_OK; ++t) rc = adjoint_recompute_sweep(ad, t, t - s0);
+        for (int64_t t = e - 1; t >= s0 && rc == PIC_OK; --t) {
+            rc = adjoint_step(ad, t, t - s0, cur, pend);
+            cur ^= 1;
+        }
+    }
+    if (rc) return rc;
+    if (pend.out) {
+        hipLaunchKernelGGL(pic::fold_rows_kernel<pic::kAdjNV>, dim3(1), dim3(256), 0, s->stream, pend.part, pend.n,
+                           pic::kAdjStride, pend.out);
+        HIP_LAUNCH_CHECK();
+    }
+    HIP_TRY(hipEventRecord(ad->ev[2], s->stream));
+    // per-step records to the host, summed there in step order
+    std::vector<double> per(T * 4), gr(T * pic::kAdjNV), di(T * 2);
+    HIP_TRY(hipMemcpyAsync(per.data(), s->per_step, T * 4 * 8, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipMemcpyAsync(gr.data(), ad->grad_rec, T * pic::kAdjNV * 8, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipMemcpyAsync(di.data(), ad->dinj_rec, T * 2 * 8, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    HIP_TRY(hipEventElapsedTime(&ad->forward_ms, ad->ev[0], ad->ev[1]));
+    HIP_TRY(hipEventElapsedTime(&ad->backward_ms, ad->ev[1], ad->ev[2]));
+    double si = 0.0, sr = 0.0;
+    for (int64_t t = 0; t < T; ++t) {
+        si += per[4 * t + 0];
+        sr += per[4 * t + 1];
+    }
+    if (sums) {
+        sums[0] = si;
+        sums[1] = sr;
+    }
+    if (d_rec)
+        for (int c = 0; c < pic::kAdjNV; ++c) {
+            double acc = 0.0;
+            for (int64_t t = 0; t < T; ++t) acc += gr[pic::kAdjNV * t + c];
+            d_rec[c] = acc;
+        }
+    if (d_inj)
+        for (int c = 0; c < 2; ++c) {
+            double acc = 0.0;
+            for (int64_t t = 0; t < T; ++t) acc += di[2 * t + c];
+            d_inj[c] = acc;
+        }
+    return PIC_OK;
+}
+
+int pic_adjoint_get_info(pic_adjoint *ad, int64_t *checkpoint_every, size_t *device_bytes, double *forward_ms,
+                         double *backward_ms) {
+    if (!ad) return fail(PIC_E_INVALID, "adjoint is NULL");
+    if (checkpoint_every) *checkpoint_every = ad->K;
+    if (device_bytes) *device_bytes = ad->bytes;
+    if (forward_ms) *forward_ms = ad->forward_ms;
+    if (backward_ms) *backward_ms = ad->backward_ms;
     return PIC_OK;
 }
 

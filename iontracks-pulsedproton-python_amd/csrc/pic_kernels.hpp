@@ -591,6 +591,10 @@ struct GatherArgs {
     double *partials;        // [n_slots][gridDim.x], scored injected charge
     int no_xy, no_z, ntj;
     double cutoff, h2b2, gf, mid, rsq;
+    // derivative variant only: per step y = blockIdx.y, the column densities'
+    // derivatives with respect to gf and h2b2 ([gridDim.y][no_xy^2] each) and
+    // the per-block scored injected-charge derivatives ([gridDim.y][gridDim.x][2])
+    double *dgf, *dh, *dpart;
 };
 
 constexpr int kGT = 16;  // gather tile edge
@@ -619,14 +623,20 @@ __device__ __forceinline__ void track_box(double x, double y, double c, double n
 // Output: dens = sum * gaussian_factor, and per block the scored injected
 // charge sum(dens * no_z) over interior columns inside the scoring disc
 // (solver_numba_parallel.py:250-277). Steps without tracks are skipped.
-__global__ void __launch_bounds__(256) gather_kernel(GatherArgs a) {
+//
+// DERIV (adjoint only) also forms, per column, dd/dgf = sum gi gj and
+// dd/dh2b2 = -gf sum gi gj (di^2 + dj^2), di = i - x, the factors being
+// exp(-di^2 h2b2) and exp(-dj^2 h2b2).
+template <bool DERIV>
+__device__ __forceinline__ void gather_body(const GatherArgs &a) {
     __shared__ double s_x[256], s_y[256];  // centres of this chunk's tracks that reach the tile
     __shared__ int s_wcnt[4];
     __shared__ double s_g[kGB][2 * kGT];
+    __shared__ double s_d2[DERIV ? kGB : 1][2 * kGT];  // squared offsets (DERIV)
     const int64_t step = a.first_step + blockIdx.y;
     const int64_t t_begin = a.offsets[step];
     const int64_t m = a.offsets[step + 1] - t_begin;
-    if (m <= 0) return;
+    if (!DERIV && m <= 0) return;  // (the derivative variant writes its zeros)
     const double *xs = a.xs + t_begin;
     const double *ys = a.ys + t_begin;
     const int64_t slot = step % a.n_slots;
@@ -636,7 +646,7 @@ __global__ void __launch_bounds__(256) gather_kernel(GatherArgs a) {
     const int ty = tid / kGT, tx = tid % kGT;
     const int i = ti0 + ty, j = tj0 + tx;
     const double nx = (double)a.no_xy;
-    double acc = 0.0;
+    double acc = 0.0, acc2 = 0.0;
     for (int64_t base = 0; base < m; base += 256) {
         const int64_t t = base + tid;
         bool hit = false;
@@ -674,23 +684,49 @@ __global__ void __launch_bounds__(256) gather_kernel(GatherArgs a) {
                 const bool in_box = f < kGT ? (cell >= li && cell < hi) : (cell >= lj && cell < hj);
                 const double d = f < kGT ? (double)cell - x : (double)cell - y;
                 s_g[q][f] = in_box ? exp(-(d * d) * a.h2b2) : 0.0;
+                if (DERIV) s_d2[q][f] = d * d;
             }
             __syncthreads();
             for (int q2 = 0; q2 < nq; ++q2) acc += s_g[q2][ty] * s_g[q2][kGT + tx];
+            if (DERIV)
+                for (int q2 = 0; q2 < nq; ++q2)
+                    acc2 += (s_g[q2][ty] * s_g[q2][kGT + tx]) * (s_d2[q2][ty] + s_d2[q2][kGT + tx]);
             __syncthreads();
         }
     }
     double inj = 0.0, z1 = 0.0, z2 = 0.0;
-    if (i < a.no_xy && j < a.no_xy) {
+    if (!DERIV && i < a.no_xy && j < a.no_xy) {
         const double d = acc * a.gf;
         dens[(int64_t)i * a.no_xy + j] = d;
         const double di = (double)i - a.mid, dj = (double)j - a.mid;
         if (acc != 0.0 && i >= 1 && i <= a.no_xy - 2 && j >= 1 && j <= a.no_xy - 2 && di * di + dj * dj < a.rsq)
             inj = d * (double)a.no_z;
     }
-    block_sum3(inj, z1, z2);
-    if (tid == 0) a.partials[slot * gridDim.x + blockIdx.x] = inj;
+    if constexpr (DERIV) {
+        double e_gf = 0.0, e_h = 0.0, z3 = 0.0;
+        if (i < a.no_xy && j < a.no_xy) {
+            const int64_t col = (int64_t)blockIdx.y * a.no_xy * a.no_xy + (int64_t)i * a.no_xy + j;
+            const double dh = -a.gf * acc2;
+            a.dgf[col] = acc;
+            a.dh[col] = dh;
+            const double di = (double)i - a.mid, dj = (double)j - a.mid;
+            if (i >= 1 && i <= a.no_xy - 2 && j >= 1 && j <= a.no_xy - 2 && di * di + dj * dj < a.rsq) {
+                e_gf = acc * (double)a.no_z;
+                e_h = dh * (double)a.no_z;
+            }
+        }
+        block_sum3(e_gf, e_h, z3);
+        if (tid == 0) {
+            a.dpart[2 * ((int64_t)blockIdx.y * gridDim.x + blockIdx.x) + 0] = e_gf;
+            a.dpart[2 * ((int64_t)blockIdx.y * gridDim.x + blockIdx.x) + 1] = e_h;
+        }
+    } else {
+        block_sum3(inj, z1, z2);
+        if (tid == 0) a.partials[slot * gridDim.x + blockIdx.x] = inj;
+    }
 }
+
+__global__ void __launch_bounds__(256) gather_kernel(GatherArgs a) { gather_body<false>(a); }
 
 // Explicit deposition into the current arrays (kernel-level API): add the
 // step's 2D density to every gap layer (solver_numba_parallel.py:253-262).

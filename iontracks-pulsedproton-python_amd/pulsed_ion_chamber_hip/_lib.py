@@ -56,6 +56,10 @@ EXPORTED = (
     "pic_batch_synchronize",
     "pic_batch_enable_timing",
     "pic_batch_get_timing",
+    "pic_adjoint_create",
+    "pic_adjoint_destroy",
+    "pic_adjoint_run",
+    "pic_adjoint_get_info",
 )
 
 
@@ -137,6 +141,10 @@ def open_library():
         "pic_batch_synchronize": [_VP],
         "pic_batch_enable_timing": [_VP, ctypes.c_int],
         "pic_batch_get_timing": [_VP, _DP, _IP, _DP, _IP],
+        "pic_adjoint_create": [_VP, ctypes.c_int64, ctypes.POINTER(_VP)],
+        "pic_adjoint_destroy": [_VP],
+        "pic_adjoint_run": [_VP, _DP, _DP, _DP],
+        "pic_adjoint_get_info": [_VP, _IP, ctypes.POINTER(ctypes.c_size_t), _DP, _DP],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

@@ -56,14 +56,16 @@ class HipSimulation:
     tests/test_charge_conservation.py drives its Numba kernels.
     """
 
-    def __init__(self, config, precision: str = "fp64"):
+    def __init__(self, config, precision: str = "fp64", params=None):
         if precision not in _lib.PRECISIONS:
             raise ValueError(f'precision must be "fp64" or "fp32", got {precision!r}.')
         self.L = _lib.open_library()
         _lib.device_info()
         self.config = config
         self.precision = precision
-        self.params = params_from_config(config)
+        # ``params`` (a PicParams) overrides the weights, coefficients and
+        # deposition constants derived from ``config`` (the grid must agree)
+        self.params = params if params is not None else params_from_config(config)
         handle = ctypes.c_void_p()
         _lib.check(self.L.pic_create(ctypes.byref(self.params), _lib.PRECISIONS[precision], ctypes.byref(handle)),
                    "pic_create")
