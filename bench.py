@@ -239,12 +239,56 @@ def curve_bench(precision, cpu_budget_s, warmup=20):
     return out
 
 
+def adjoint_bench(fd: bool = True):
+    """dk_s/dtheta on the 10 um, r 0.02 cm grid over 3 pulses (FP64 fields,
+    checkpoints every 64 steps): forward and reverse time, device memory, and
+    the gradient against FP64 central finite differences (relative step 1e-6,
+    dt and step count frozen) on all six parameters."""
+    from configs import readme_config
+    from pulsed_ion_chamber_hip.adjoint import PARAMS, KsModel
+
+    cfg = readme_config(10.0, 0.02, n_pulses=3)
+    model = KsModel(cfg, checkpoint_every=64)
+    model.ks_and_gradient()  # warm-up (kernels loaded, memory touched)
+    t0 = time.perf_counter()
+    ks, grad = model.ks_and_gradient()
+    wall = time.perf_counter() - t0
+    info = dict(model.last_info)
+    t0 = time.perf_counter()
+    ks_plain = model.ks(model.theta0)
+    plain = time.perf_counter() - t0
+    out = {
+        "grid": [cfg.no_xy, cfg.no_xy, cfg.no_z_with_buffer],
+        "steps": cfg.total_time_steps,
+        "ks": ks,
+        "ks_forward_only": ks_plain,
+        "dks_dtheta": dict(zip(PARAMS, grad.tolist())),
+        "theta": dict(zip(PARAMS, model.theta0.tolist())),
+        "forward_with_checkpoints_s": info["forward_ms"] / 1e3,
+        "reverse_s": info["backward_ms"] / 1e3,
+        "reverse_over_forward": info["backward_ms"] / info["forward_ms"],
+        "gradient_wall_s": wall,
+        "plain_forward_wall_s": plain,
+        "checkpoint_every": info["checkpoint_every"],
+        "adjoint_device_bytes": info["device_bytes"],
+        "plain_state_bytes": 4 * cfg.no_xy**2 * cfg.no_z_with_buffer * 8,
+    }
+    out["memory_over_plain"] = out["adjoint_device_bytes"] / out["plain_state_bytes"]
+    if fd:
+        t0 = time.perf_counter()
+        fdv = model.finite_difference(rel=1e-6)
+        out["fd_s"] = time.perf_counter() - t0
+        out["fd_dks_dtheta"] = dict(zip(PARAMS, fdv.tolist()))
+        out["max_rel_err_vs_fd"] = float(np.max(np.abs(grad - fdv) / np.abs(fdv)))
+    return cfg, out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=0, help="timed steps (default: the whole run)")
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--workload", default="c1", choices=["c0", "c1", "c2", "c4"])
+    ap.add_argument("--workload", default="c1", choices=["c0", "c1", "c2", "c3", "c4"])
     ap.add_argument("--no-curve", action="store_true", help="skip the saturation-curve comparison")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
     ap.add_argument("--no-reference-precision", action="store_true", help="skip the FP64 comparison run")
@@ -298,6 +342,31 @@ def main():
                 "sample": f"first {co['configs']} of 32 curve configurations, whole runs, FP64 fields, "
                           f"{co['threads']} OpenMP threads, {co['seconds']:.1f} s",
             }
+        print(json.dumps(out))
+        return
+    if args.workload == "c3":
+        print(f"bench: adjoint dk_s/dtheta, 10 um r 0.02 cm, 3 pulses, fp64, device {device}", file=sys.stderr,
+              flush=True)
+        cfg, adj = adjoint_bench(fd=not args.no_cpu)
+        nvox = interior_voxels(cfg)
+        total_s = adj["forward_with_checkpoints_s"] + adj["reverse_s"]
+        out = {
+            "metric": METRIC,
+            "value": nvox * cfg.total_time_steps / adj["forward_with_checkpoints_s"],
+            "unit": "voxel-steps/s",
+            "n_gpus": 1,
+            "steps": cfg.total_time_steps,
+            "warmup": 1,
+            "ms_per_step": total_s / cfg.total_time_steps * 1e3,
+            "higher_is_better": True,
+            "scaling": "weak",
+            "vs_baseline": None,
+            "dtype": "f64",
+            "data": "synthetic",
+            "config": {"workload": "adjoint_10um_r0.02cm_3pulses", "grid": adj["grid"], "fields": "fp64",
+                       "checkpoint_every": adj["checkpoint_every"]},
+            "adjoint": adj,
+        }
         print(json.dumps(out))
         return
     cfg, wl_name = workload(args.workload)
