@@ -239,7 +239,7 @@ def curve_bench(precision, cpu_budget_s, warmup=20):
     return out
 
 
-def adjoint_bench(fd: bool = True):
+def adjoint_bench(fd: bool = True, pulses: int = 3):
     """dk_s/dtheta on the 10 um, r 0.02 cm grid over 3 pulses (FP64 fields,
     checkpoints every 64 steps): forward and reverse time, device memory, and
     the gradient against FP64 central finite differences (relative step 1e-6,
@@ -247,7 +247,7 @@ def adjoint_bench(fd: bool = True):
     from configs import readme_config
     from pulsed_ion_chamber_hip.adjoint import PARAMS, KsModel
 
-    cfg = readme_config(10.0, 0.02, n_pulses=3)
+    cfg = readme_config(10.0, 0.02, n_pulses=pulses)
     model = KsModel(cfg, checkpoint_every=64)
     model.ks_and_gradient()  # warm-up (kernels loaded, memory touched)
     t0 = time.perf_counter()
@@ -290,6 +290,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--workload", default="c1", choices=["c0", "c1", "c2", "c3", "c4"])
     ap.add_argument("--no-curve", action="store_true", help="skip the saturation-curve comparison")
+    ap.add_argument("--adjoint-pulses", type=int, default=3, help="pulses of the adjoint workload (c3)")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
     ap.add_argument("--no-reference-precision", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--cpu-budget-s", type=float, default=12.0)
@@ -347,7 +348,7 @@ def main():
     if args.workload == "c3":
         print(f"bench: adjoint dk_s/dtheta, 10 um r 0.02 cm, 3 pulses, fp64, device {device}", file=sys.stderr,
               flush=True)
-        cfg, adj = adjoint_bench(fd=not args.no_cpu)
+        cfg, adj = adjoint_bench(fd=not args.no_cpu, pulses=args.adjoint_pulses)
         nvox = interior_voxels(cfg)
         total_s = adj["forward_with_checkpoints_s"] + adj["reverse_s"]
         out = {
@@ -363,7 +364,7 @@ def main():
             "vs_baseline": None,
             "dtype": "f64",
             "data": "synthetic",
-            "config": {"workload": "adjoint_10um_r0.02cm_3pulses", "grid": adj["grid"], "fields": "fp64",
+            "config": {"workload": f"adjoint_10um_r0.02cm_{args.adjoint_pulses}pulses", "grid": adj["grid"], "fields": "fp64",
                        "checkpoint_every": adj["checkpoint_every"]},
             "adjoint": adj,
         }

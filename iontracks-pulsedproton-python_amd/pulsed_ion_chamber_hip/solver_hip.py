@@ -190,6 +190,7 @@ def run_simulation_hip(
     return_fields: bool = True,
     precision: str = "fp64",
     chunk_steps: int = 256,
+    tiling=None,
 ):
     """Simulate a pulse train on the GPU and return the reference's Result.
 
@@ -203,6 +204,12 @@ def run_simulation_hip(
     sweeps FP32 fields (half the HBM traffic) while deposition, the
     recombination product and every scored sum stay FP64.
 
+    ``tiling`` picks the sweep's tile shape: None for the library's rule,
+    ``"auto"`` for the autotuned shape (measured once per device, grid,
+    precision and wall, then cached; see :mod:`.tuning`), or
+    ``(rows_per_tile, planes_per_chunk, values_per_lane)``. Every tiling
+    gives the same fields.
+
     ``max_wall_s`` / ``max_steps`` stop early exactly as in
     run_simulation_cuda (solver_cuda.py:664-671): ``ks`` is NaN and
     ``steps_completed`` / ``loop_elapsed_s`` are set. ``return_fields=False``
@@ -215,10 +222,18 @@ def run_simulation_hip(
     _lib.open_library()
     _lib.device_info()
 
+    if tiling == "auto":
+        from .tuning import autotune
+
+        tiling = tuple(autotune(config, precision)["tiling"])
+    elif tiling is not None and (isinstance(tiling, str) or len(tiling) != 3):
+        raise ValueError(f'tiling must be None, "auto" or (rows, planes, values), got {tiling!r}.')
     schedule, xs, ys = draw_tracks(config, rng)
     total = config.total_time_steps
     last = total if max_steps is None else min(total, int(max_steps))
     with HipSimulation(config, precision) as sim:
+        if tiling is not None:
+            sim.set_tiling(*tiling)
         sim.upload_tracks(schedule, xs, ys)
         report_every = max(1, total // 20)
         loop_t0 = perf_counter()
