@@ -36,11 +36,11 @@ AIC_KWARGS = dict(
 
 
 def aic_config(radius_cm, grid_um, dose_rate_water_Gy_s=10.0, **overrides):
-    kw = dict(AIC_KWARGS)
+    kw = dict(AIC_KWARGS, max_voxels=2e9, memory_budget_fraction=None)
     kw.update(overrides)
     return SimulationConfig(
         **kw, dose_rate_Gy_s=dose_rate_water_Gy_s * 0.891, grid_size_um=grid_um, sampled_radius_cm=radius_cm,
-        buffer_radius=3, no_z_electrode=5, max_voxels=2e9, memory_budget_fraction=None,
+        buffer_radius=3, no_z_electrode=5,
     )
 
 
