@@ -221,6 +221,6 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjointArgs a) {
 }
 
 // The derivative variant of the deposition gather (dd/dgf, dd/dh2b2).
-__global__ void __launch_bounds__(256) gather_deriv_kernel(GatherArgs a) { gather_body<true>(a); }
+__global__ void __launch_bounds__(256) gather_deriv_kernel(GatherArgs a) { gather_body<true>(a, blockIdx.x, blockIdx.y); }
 
 }  // namespace pic

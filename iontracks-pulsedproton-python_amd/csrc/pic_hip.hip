@@ -84,6 +84,9 @@ struct pic_sim {
     int64_t n_tracks = 0;
     std::vector<int64_t> offsets;
     int force_tj = 0, force_chunk = 0, force_v1 = 0;
+    int cus = 256;                 // compute units of the device (queried once)
+    bool shape_valid = false;      // `shape` matches the tiling knobs
+    SweepShape shape;
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;  // reused across calls
     std::vector<int> ev_kind;         // per recorded event: 0 sweep, 1 deposit, 2 ring + reduce
@@ -112,8 +115,15 @@ struct pic_batch {
     int64_t slot_steps = 0, slot_cfgs = 0;
     pic::SweepArgs *descs_h[2] = {nullptr, nullptr}, *descs_d[2] = {nullptr, nullptr};
     int *offs_h[2] = {nullptr, nullptr}, *offs_d[2] = {nullptr, nullptr};
+    // the segment's gathers: one GatherArgs per configuration with tracks, and block offsets
+    pic::GatherArgs *gdesc_h[2] = {nullptr, nullptr}, *gdesc_d[2] = {nullptr, nullptr};
+    int *goffs_h[2] = {nullptr, nullptr}, *goffs_d[2] = {nullptr, nullptr};
     hipEvent_t slot_ev[2] = {nullptr, nullptr};
     bool slot_busy[2] = {false, false};
+    double *zero_dens = nullptr;  // the density of a step without tracks, no_xy^2 of the largest grid
+    std::vector<pic::SweepArgs> descs_build;  // host-side build of one segment's tables
+    std::vector<int> offs_build, goffs_build;
+    std::vector<pic::GatherArgs> gdesc_build;
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;  // pairs around each launch: kind 0 sweep, 1 gather
     std::vector<int> ev_kind;
@@ -171,7 +181,18 @@ SweepShape fixed_shape(const pic_sim *s, int v, int tj, int chunk, int wpb) {
 
 // Rows per tile (registers) and planes per block: enough blocks to keep every
 // CU busy, as few halo rows and chunk prologues as that allows.
-SweepShape sweep_shape(const pic_sim *s) {
+SweepShape compute_sweep_shape(const pic_sim *s);
+
+// the handle's tiling, recomputed only when a knob changes
+SweepShape sweep_shape(pic_sim *s) {
+    if (!s->shape_valid) {
+        s->shape = compute_sweep_shape(s);
+        s->shape_valid = true;
+    }
+    return s->shape;
+}
+
+SweepShape compute_sweep_shape(const pic_sim *s) {
     SweepShape sh;
     const int64_t no_xy = s->p.no_xy, nzb = s->p.nzb;
     sh.v = values_per_lane(s, s->force_v1);
@@ -179,7 +200,7 @@ SweepShape sweep_shape(const pic_sim *s) {
     sh.wpb = std::min(sh.n_windows, 8);
     sh.n_kg = (int)ceil_div(sh.n_windows, sh.wpb);
     const int64_t rows = no_xy - 2, planes = no_xy - 2;
-    const int cus = sm_count(s->device);
+    const int cus = s->cus;
     // Enough waves for ~16 per CU, with the tallest tile (fewest halo rows) and
     // the longest chunk (fewest prologue planes) that still get there. Measured
     // on MI355X (in-pulse steps): FP32 fields best at 4-row tiles on the
@@ -398,10 +419,10 @@ int64_t gather_blocks(const pic_sim *s) {
     return nt * nt;
 }
 
-// deposition densities of steps [first, first+count) (count <= batch) into
-// dens[0..count) and dep_part[0..count); offsets are device offsets into xs/ys
-int enqueue_gather(pic_sim *s, const double *xs, const double *ys, const int64_t *offsets, int64_t first,
-                   int64_t count, hipStream_t st = nullptr) {
+// arguments of the gather of steps from `first` (slot = step mod (batch+1));
+// offsets are device offsets into xs/ys
+pic::GatherArgs make_gather_args(const pic_sim *s, const double *xs, const double *ys, const int64_t *offsets,
+                                 int64_t first) {
     pic::GatherArgs g{};
     g.xs = xs;
     g.ys = ys;
@@ -418,6 +439,14 @@ int enqueue_gather(pic_sim *s, const double *xs, const double *ys, const int64_t
     g.gf = s->p.gaussian_factor;
     g.mid = s->p.mid_xy;
     g.rsq = s->p.scoring_radius_sq;
+    return g;
+}
+
+// deposition densities of steps [first, first+count) (count <= batch) into
+// their dens and dep_part slots
+int enqueue_gather(pic_sim *s, const double *xs, const double *ys, const int64_t *offsets, int64_t first,
+                   int64_t count, hipStream_t st = nullptr) {
+    const pic::GatherArgs g = make_gather_args(s, xs, ys, offsets, first);
     record(s, 1);
     hipLaunchKernelGGL(pic::gather_kernel, dim3((unsigned)gather_blocks(s), (unsigned)count), dim3(256), 0,
                        st ? st : s->stream, g);
@@ -482,7 +511,7 @@ int batch_tiling(pic_batch *b) {
     for (pic_sim *s : b->sims) v = std::min(v, values_per_lane(s, false));
     for (pic_sim *s : b->sims)
         wpb = std::min(wpb, (int)ceil_div(s->p.nzb - 2, pic::kWave * v - 2));
-    const int cus = sm_count(b->sims[0]->device);
+    const int cus = b->sims[0]->cus;
     std::vector<std::pair<int, int>> cands;
     if (b->force_tj > 0 || b->force_chunk > 0)
         cands = {{b->force_tj > 0 ? b->force_tj : 1, b->force_chunk > 0 ? b->force_chunk : 4}};
@@ -509,10 +538,16 @@ void free_batch_slots(pic_batch *b) {
     for (int k = 0; k < 2; ++k) {
         if (b->descs_h[k]) hipHostFree(b->descs_h[k]);
         if (b->offs_h[k]) hipHostFree(b->offs_h[k]);
+        if (b->gdesc_h[k]) hipHostFree(b->gdesc_h[k]);
+        if (b->goffs_h[k]) hipHostFree(b->goffs_h[k]);
         if (b->descs_d[k]) hipFree(b->descs_d[k]);
         if (b->offs_d[k]) hipFree(b->offs_d[k]);
+        if (b->gdesc_d[k]) hipFree(b->gdesc_d[k]);
+        if (b->goffs_d[k]) hipFree(b->goffs_d[k]);
         b->descs_h[k] = b->descs_d[k] = nullptr;
         b->offs_h[k] = b->offs_d[k] = nullptr;
+        b->gdesc_h[k] = b->gdesc_d[k] = nullptr;
+        b->goffs_h[k] = b->goffs_d[k] = nullptr;
         b->slot_busy[k] = false;
     }
     b->slot_steps = b->slot_cfgs = 0;
@@ -527,7 +562,11 @@ int ensure_batch_slots(pic_batch *b, int64_t steps) {
         if (hipHostMalloc((void **)&b->descs_h[k], (size_t)(steps * n) * sizeof(pic::SweepArgs)) != hipSuccess ||
             hipHostMalloc((void **)&b->offs_h[k], (size_t)(steps * (n + 1)) * sizeof(int)) != hipSuccess ||
             hipMalloc(&b->descs_d[k], (size_t)(steps * n) * sizeof(pic::SweepArgs)) != hipSuccess ||
-            hipMalloc(&b->offs_d[k], (size_t)(steps * (n + 1)) * sizeof(int)) != hipSuccess) {
+            hipMalloc(&b->offs_d[k], (size_t)(steps * (n + 1)) * sizeof(int)) != hipSuccess ||
+            hipHostMalloc((void **)&b->gdesc_h[k], (size_t)n * sizeof(pic::GatherArgs)) != hipSuccess ||
+            hipHostMalloc((void **)&b->goffs_h[k], (size_t)(n + 1) * sizeof(int)) != hipSuccess ||
+            hipMalloc(&b->gdesc_d[k], (size_t)n * sizeof(pic::GatherArgs)) != hipSuccess ||
+            hipMalloc(&b->goffs_d[k], (size_t)(n + 1) * sizeof(int)) != hipSuccess) {
             free_batch_slots(b);
             return fail(PIC_E_NOMEM, "allocation of the batch argument tables failed");
         }
@@ -714,6 +753,7 @@ int pic_create(const pic_params *p, int precision, pic_sim **out) {
     s->pitch = row_pitch(p, precision);
     s->n_vox = p->no_xy * p->no_xy * s->pitch;
     HIP_TRY(hipGetDevice(&s->device));
+    s->cus = sm_count(s->device);
     auto cleanup = [&](int code) {
         pic_destroy(s);
         return code;
@@ -967,11 +1007,13 @@ int pic_set_tiling(pic_sim *s, int rows_per_tile, int planes_per_chunk, int valu
     s->force_v1 = values_per_lane == 1;
     s->force_tj = rows_per_tile;
     s->force_chunk = planes_per_chunk;
+    s->shape_valid = false;
     const int rc = ensure_partials(s, sweep_shape(s).blocks());
     if (rc) {
         s->force_tj = old_tj;
         s->force_chunk = old_chunk;
         s->force_v1 = old_v1;
+        s->shape_valid = false;
     }
     return rc;
 }
@@ -1042,6 +1084,7 @@ int pic_batch_create(pic_sim *const *sims, int n, pic_batch **out) {
     if (dev != sims[0]->device) return fail(PIC_E_INVALID, "the handles live on device %d, current is %d", sims[0]->device, dev);
     pic_batch *b = new pic_batch();
     b->sims.assign(sims, sims + n);
+    b->gdesc_build.resize(n);
     b->prec = sims[0]->prec;
     b->reflect = sims[0]->p.wall == PIC_WALL_REFLECTING;
     auto cleanup = [&](int code) {
@@ -1050,6 +1093,11 @@ int pic_batch_create(pic_sim *const *sims, int n, pic_batch **out) {
     };
     int rc = batch_tiling(b);
     if (rc) return cleanup(rc);
+    int64_t xy_max = 0;
+    for (int c = 0; c < n; ++c) xy_max = std::max(xy_max, sims[c]->p.no_xy * sims[c]->p.no_xy);
+    if (hipMalloc(&b->zero_dens, (size_t)xy_max * 8) != hipSuccess ||
+        hipMemset(b->zero_dens, 0, (size_t)xy_max * 8) != hipSuccess)
+        return cleanup(fail(PIC_E_NOMEM, "allocation of the batch's zero density failed"));
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&b->done, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&b->slot_ev[0], hipEventDisableTiming) != hipSuccess ||
@@ -1063,6 +1111,7 @@ int pic_batch_destroy(pic_batch *b) {
     if (!b) return PIC_OK;
     if (b->stream) hipStreamSynchronize(b->stream);
     free_batch_slots(b);
+    if (b->zero_dens) hipFree(b->zero_dens);
     for (hipEvent_t e : b->slot_ev)
         if (e) hipEventDestroy(e);
     if (b->done) hipEventDestroy(b->done);
@@ -1131,9 +1180,11 @@ int pic_batch_run(pic_batch *b, int64_t first, int64_t count) {
     // one segment per gather batch: gathers first, then the steps' sweeps
     for (int64_t s0 = first; s0 < first + count && rc == PIC_OK; s0 += gbatch, k ^= 1) {
         const int64_t s1 = std::min(first + count, s0 + gbatch);
-        if (b->slot_busy[k]) HIP_TRY(hipEventSynchronize(b->slot_ev[k]));  // host slot k copied out
-        pic::SweepArgs *descs = b->descs_h[k];
-        int *offs = b->offs_h[k];
+        // built in ordinary host memory, copied into the pinned slot in one pass
+        b->descs_build.resize((size_t)(s1 - s0) * n);
+        b->offs_build.resize((size_t)(s1 - s0) * (n + 1));
+        pic::SweepArgs *descs = b->descs_build.data();
+        int *offs = b->offs_build.data();
         active.assign(s1 - s0, 0);
         cand_of.assign(s1 - s0, 0);
         // descriptors of every (step, configuration) of the segment, built in
@@ -1159,6 +1210,7 @@ int pic_batch_run(pic_batch *b, int64_t first, int64_t count) {
                 const bool dep = s->offsets[t + 1] > s->offsets[t];
                 rc = make_sweep_args(s, sh, t, dep, s->dens + slot * xy, s->dep_part + slot * n_tiles, n_tiles,
                                      s->per_step + 4 * t, descs[(t - s0) * n + q]);
+                if (!dep) descs[(t - s0) * n + q].dens = b->zero_dens;  // the batched kernel always deposits
                 swap_buffers(s);
                 const int64_t nxt = (int64_t)o[q] + sh.blocks();
                 if (nxt > INT32_MAX / 2) rc = fail(PIC_E_INVALID, "batch grid too large");
@@ -1167,21 +1219,45 @@ int pic_batch_run(pic_batch *b, int64_t first, int64_t count) {
             active[t - s0] = na;
         }
         if (rc) break;
-        HIP_TRY(hipMemcpyAsync(b->descs_d[k], descs, (size_t)((s1 - s0) * n) * sizeof(pic::SweepArgs),
-                               hipMemcpyHostToDevice, b->stream));
-        HIP_TRY(hipMemcpyAsync(b->offs_d[k], offs, (size_t)((s1 - s0) * (n + 1)) * sizeof(int),
-                               hipMemcpyHostToDevice, b->stream));
-        HIP_TRY(hipEventRecord(b->slot_ev[k], b->stream));
-        b->slot_busy[k] = true;
-        for (int q = 0; q < n && rc == PIC_OK; ++q) {
+        if (b->slot_busy[k]) HIP_TRY(hipEventSynchronize(b->slot_ev[k]));  // host slot k copied out
+        const size_t dbytes = (size_t)((s1 - s0) * n) * sizeof(pic::SweepArgs);
+        const size_t obytes = (size_t)((s1 - s0) * (n + 1)) * sizeof(int);
+        memcpy(b->descs_h[k], descs, dbytes);
+        memcpy(b->offs_h[k], offs, obytes);
+        HIP_TRY(hipMemcpyAsync(b->descs_d[k], b->descs_h[k], dbytes, hipMemcpyHostToDevice, b->stream));
+        HIP_TRY(hipMemcpyAsync(b->offs_d[k], b->offs_h[k], obytes, hipMemcpyHostToDevice, b->stream));
+        // one launch gathers the segment's steps of every configuration with tracks
+        int ng = 0;
+        std::vector<int> &goffs = b->goffs_build;
+        goffs.assign(n + 1, 0);
+        for (int q = 0; q < n; ++q) {
             pic_sim *s = b->sims[ord[q]];
             const int64_t e = std::min(s1, s->n_steps);
             if (s0 >= e) break;
             if (s->offsets[e] > s->offsets[s0]) {
-                record_batch(b, 1);
-                rc = enqueue_gather(s, s->xs, s->ys, s->offsets_d, s0, e - s0, b->stream);
-                record_batch(b, 1);
+                b->gdesc_build[ng] = make_gather_args(s, s->xs, s->ys, s->offsets_d, s0);
+                const int64_t nxt = goffs[ng] + gather_blocks(s) * (e - s0);
+                if (nxt > INT32_MAX / 2) return fail(PIC_E_INVALID, "batch gather grid too large");
+                goffs[ng + 1] = (int)nxt;
+                ++ng;
             }
+        }
+        if (ng > 0) {
+            memcpy(b->gdesc_h[k], b->gdesc_build.data(), ng * sizeof(pic::GatherArgs));
+            memcpy(b->goffs_h[k], goffs.data(), (ng + 1) * sizeof(int));
+            HIP_TRY(hipMemcpyAsync(b->gdesc_d[k], b->gdesc_h[k], ng * sizeof(pic::GatherArgs), hipMemcpyHostToDevice,
+                                   b->stream));
+            HIP_TRY(hipMemcpyAsync(b->goffs_d[k], b->goffs_h[k], (ng + 1) * sizeof(int), hipMemcpyHostToDevice,
+                                   b->stream));
+        }
+        HIP_TRY(hipEventRecord(b->slot_ev[k], b->stream));
+        b->slot_busy[k] = true;
+        if (ng > 0) {
+            record_batch(b, 1);
+            hipLaunchKernelGGL(pic::batch_gather_kernel, dim3((unsigned)goffs[ng]), dim3(256), 0, b->stream,
+                               b->gdesc_d[k], b->goffs_d[k], ng);
+            if (hipGetLastError() != hipSuccess) rc = fail(PIC_E_HIP, "batched gather launch failed");
+            record_batch(b, 1);
         }
         for (int64_t t = s0; t < s1 && rc == PIC_OK; ++t) {
             const int na = active[t - s0];

@@ -556,9 +556,11 @@ __global__ void __launch_bounds__(512) sweep_kernel(SweepArgs a) {
 // configuration b's previous step (if pending); sweep block g (after the XCD
 // remap) belongs to configuration c with offs[c] <= g < offs[c+1] and runs it
 // with that configuration's arguments descs[c] -- its arrays, weights,
-// tiling and this step's deposition (dens null when the step deposits
-// nothing). Configurations are ordered so the active ones are a prefix; the
-// grid covers exactly those.
+// tiling and this step's deposition. Every configuration takes the
+// deposition path (one compiled path keeps the register count of the
+// single-configuration kernel): one that deposits nothing this step points
+// dens at zeros, and x + 0.0 == x. Configurations are ordered so the active
+// ones are a prefix; the grid covers exactly those.
 template <typename S, typename A, int TJ, int V, bool REFLECT>
 __global__ void __launch_bounds__(512) batch_sweep_kernel(const SweepArgs *__restrict__ descs,
                                                           const int *__restrict__ offs, int n_cfg) {
@@ -578,7 +580,7 @@ __global__ void __launch_bounds__(512) batch_sweep_kernel(const SweepArgs *__res
     }
     c = __builtin_amdgcn_readfirstlane(c);
     const SweepArgs a = descs[c];
-    sweep_body<S, A, TJ, V, REFLECT>(a, g - offs[c], a.dens != nullptr);
+    sweep_body<S, A, TJ, V, REFLECT>(a, g - offs[c], true);
 }
 
 struct GatherArgs {
@@ -628,12 +630,13 @@ __device__ __forceinline__ void track_box(double x, double y, double c, double n
 // dd/dh2b2 = -gf sum gi gj (di^2 + dj^2), di = i - x, the factors being
 // exp(-di^2 h2b2) and exp(-dj^2 h2b2).
 template <bool DERIV>
-__device__ __forceinline__ void gather_body(const GatherArgs &a) {
+__device__ __forceinline__ void gather_body(const GatherArgs &a, const int tile, const int ystep) {
     __shared__ double s_x[256], s_y[256];  // centres of this chunk's tracks that reach the tile
     __shared__ int s_wcnt[4];
     __shared__ double s_g[kGB][2 * kGT];
     __shared__ double s_d2[DERIV ? kGB : 1][2 * kGT];  // squared offsets (DERIV)
-    const int64_t step = a.first_step + blockIdx.y;
+    const int n_tiles = a.ntj * a.ntj;
+    const int64_t step = a.first_step + ystep;
     const int64_t t_begin = a.offsets[step];
     const int64_t m = a.offsets[step + 1] - t_begin;
     if (!DERIV && m <= 0) return;  // (the derivative variant writes its zeros)
@@ -642,7 +645,7 @@ __device__ __forceinline__ void gather_body(const GatherArgs &a) {
     const int64_t slot = step % a.n_slots;
     double *dens = a.dens + slot * a.no_xy * a.no_xy;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int ti0 = (blockIdx.x / a.ntj) * kGT, tj0 = (blockIdx.x % a.ntj) * kGT;
+    const int ti0 = (tile / a.ntj) * kGT, tj0 = (tile % a.ntj) * kGT;
     const int ty = tid / kGT, tx = tid % kGT;
     const int i = ti0 + ty, j = tj0 + tx;
     const double nx = (double)a.no_xy;
@@ -705,7 +708,7 @@ __device__ __forceinline__ void gather_body(const GatherArgs &a) {
     if constexpr (DERIV) {
         double e_gf = 0.0, e_h = 0.0, z3 = 0.0;
         if (i < a.no_xy && j < a.no_xy) {
-            const int64_t col = (int64_t)blockIdx.y * a.no_xy * a.no_xy + (int64_t)i * a.no_xy + j;
+            const int64_t col = (int64_t)ystep * a.no_xy * a.no_xy + (int64_t)i * a.no_xy + j;
             const double dh = -a.gf * acc2;
             a.dgf[col] = acc;
             a.dh[col] = dh;
@@ -717,16 +720,37 @@ __device__ __forceinline__ void gather_body(const GatherArgs &a) {
         }
         block_sum3(e_gf, e_h, z3);
         if (tid == 0) {
-            a.dpart[2 * ((int64_t)blockIdx.y * gridDim.x + blockIdx.x) + 0] = e_gf;
-            a.dpart[2 * ((int64_t)blockIdx.y * gridDim.x + blockIdx.x) + 1] = e_h;
+            a.dpart[2 * ((int64_t)ystep * n_tiles + tile) + 0] = e_gf;
+            a.dpart[2 * ((int64_t)ystep * n_tiles + tile) + 1] = e_h;
         }
     } else {
         block_sum3(inj, z1, z2);
-        if (tid == 0) a.partials[slot * gridDim.x + blockIdx.x] = inj;
+        if (tid == 0) a.partials[slot * n_tiles + tile] = inj;
     }
 }
 
-__global__ void __launch_bounds__(256) gather_kernel(GatherArgs a) { gather_body<false>(a); }
+__global__ void __launch_bounds__(256) gather_kernel(GatherArgs a) { gather_body<false>(a, blockIdx.x, blockIdx.y); }
+
+// The gathers of many configurations in one launch: block g belongs to
+// configuration c with offs[c] <= g < offs[c+1] (c found by one ballot over
+// the offsets) and runs tile (g - offs[c]) % tiles of step (g - offs[c]) /
+// tiles of that configuration's arguments descs[c].
+__global__ void __launch_bounds__(256) batch_gather_kernel(const GatherArgs *__restrict__ descs,
+                                                           const int *__restrict__ offs, int n_cfg) {
+    const int g = blockIdx.x;
+    const int lane = threadIdx.x & 63;
+    int c = 0;
+    for (int base = 1; base < n_cfg; base += kWave) {
+        const int q = base + lane;
+        c += __popcll(__ballot(q < n_cfg && offs[q] <= g));
+    }
+    // every wave of the block computed the same c; make it uniform
+    c = __builtin_amdgcn_readfirstlane(c);
+    const GatherArgs a = descs[c];
+    const int local = g - offs[c];
+    const int tiles = a.ntj * a.ntj;
+    gather_body<false>(a, local % tiles, local / tiles);
+}
 
 // Explicit deposition into the current arrays (kernel-level API): add the
 // step's 2D density to every gap layer (solver_numba_parallel.py:253-262).
