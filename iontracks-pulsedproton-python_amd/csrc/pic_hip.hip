@@ -142,10 +142,11 @@ struct pic_adjoint {
     double *mu = nullptr;     // [2 buffers][2 species][n_vox]
     double *dgf = nullptr, *dh = nullptr;  // [K][no_xy^2]
     double *dpart = nullptr;               // [K][gather tiles][2]
-    double *apart = nullptr;               // [2][adjoint blocks][kAdjStride]
-    double *grad_rec = nullptr;            // [T][kAdjNV]
+    double *acc = nullptr;                 // [adjoint blocks][2 * kAdjNV] running compensated sums
+    double *grad = nullptr;                // [kAdjNV] folded totals
     double *dinj_rec = nullptr;            // [T][2]
     double *scratch = nullptr;             // sweep records of the recompute (discarded)
+    double *zeros = nullptr;               // [no_xy^2] the density (and derivatives) of a step without tracks
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     float forward_ms = 0, backward_ms = 0;
     size_t bytes = 0;
@@ -638,14 +639,8 @@ int adjoint_deriv_gather(pic_adjoint *ad, int64_t first, int64_t count) {
     return PIC_OK;
 }
 
-struct AdjPending {
-    const double *part = nullptr;
-    int n = 0;
-    double *out = nullptr;
-};
-
 // adjoint step t: mu_t from mu_{t+1} (buffer `cur`) into buffer cur ^ 1
-int adjoint_step(pic_adjoint *ad, int64_t t, int64_t src, int cur, AdjPending &pend) {
+int adjoint_step(pic_adjoint *ad, int64_t t, int64_t src, int cur) {
     pic_sim *s = ad->s;
     const int64_t xy = s->p.no_xy * s->p.no_xy;
     const bool dep = s->offsets[t + 1] > s->offsets[t];
@@ -654,17 +649,14 @@ int adjoint_step(pic_adjoint *ad, int64_t t, int64_t src, int cur, AdjPending &p
     const double *in = ad->store + src * 2 * s->n_vox;
     a.sp = in;
     a.sn = in + s->n_vox;
-    a.dens = dep ? s->dens + (t % (s->batch + 1)) * xy : nullptr;
-    a.dgf = dep ? ad->dgf + (t % ad->K) * xy : nullptr;
-    a.dh = dep ? ad->dh + (t % ad->K) * xy : nullptr;
+    a.dens = dep ? s->dens + (t % (s->batch + 1)) * xy : ad->zeros;
+    a.dgf = dep ? ad->dgf + (t % ad->K) * xy : ad->zeros;
+    a.dh = dep ? ad->dh + (t % ad->K) * xy : ad->zeros;
     a.mp = ad->mu + (2 * cur + 0) * s->n_vox;
     a.mn = ad->mu + (2 * cur + 1) * s->n_vox;
     a.op = ad->mu + (2 * (cur ^ 1) + 0) * s->n_vox;
     a.on = ad->mu + (2 * (cur ^ 1) + 1) * s->n_vox;
-    a.partials = ad->apart + (t % 2) * nblk * pic::kAdjStride;
-    a.prev_part = pend.part;
-    a.prev_n = pend.n;
-    a.prev_out = pend.out;
+    a.acc = ad->acc;
     a.no_xy = (int)s->p.no_xy;
     a.nzb = (int)s->p.nzb;
     a.nze = (int)s->p.nze;
@@ -683,13 +675,12 @@ int adjoint_step(pic_adjoint *ad, int64_t t, int64_t src, int cur, AdjPending &p
     a.n_zp = s->p.n_zp;
     a.n_cen = s->p.n_cen;
     a.adt = s->p.alpha_dt;
-    const dim3 grid((unsigned)(nblk + (pend.out ? 1 : 0))), block(256);
+    const dim3 grid((unsigned)nblk), block(256);
     if (s->p.wall == PIC_WALL_REFLECTING)
         hipLaunchKernelGGL(pic::adjoint_kernel<true>, grid, block, 0, s->stream, a);
     else
         hipLaunchKernelGGL(pic::adjoint_kernel<false>, grid, block, 0, s->stream, a);
     HIP_LAUNCH_CHECK();
-    pend = AdjPending{a.partials, (int)nblk, ad->grad_rec + pic::kAdjNV * t};
     return PIC_OK;
 }
 
@@ -1333,10 +1324,10 @@ int pic_adjoint_create(pic_sim *s, int64_t checkpoint_every, pic_adjoint **out) 
     const size_t pair = 2 * (size_t)s->n_vox * 8;
     const size_t sizes[] = {ad->nseg * pair, (size_t)(ad->K + 1) * pair, 2 * pair, (size_t)ad->K * xy * 8,
                             (size_t)ad->K * xy * 8, (size_t)ad->K * tiles * 2 * 8,
-                            (size_t)2 * nblk * pic::kAdjStride * 8, (size_t)ad->T * pic::kAdjNV * 8,
-                            (size_t)ad->T * 2 * 8, 64};
+                            (size_t)nblk * 2 * pic::kAdjNV * 8, (size_t)pic::kAdjNV * 8,
+                            (size_t)ad->T * 2 * 8, 64, (size_t)xy * 8};
     double **ptrs[] = {&ad->ckpt, &ad->store, &ad->mu, &ad->dgf, &ad->dh, &ad->dpart,
-                       &ad->apart, &ad->grad_rec, &ad->dinj_rec, &ad->scratch};
+                       &ad->acc, &ad->grad, &ad->dinj_rec, &ad->scratch, &ad->zeros};
     size_t total = 0;
     for (size_t b : sizes) total += b;
     size_t fr = 0, tot = 0;
@@ -1372,8 +1363,8 @@ int pic_adjoint_create(pic_sim *s, int64_t checkpoint_every, pic_adjoint **out) 
 int pic_adjoint_destroy(pic_adjoint *ad) {
     if (!ad) return PIC_OK;
     if (ad->s && ad->s->stream) hipStreamSynchronize(ad->s->stream);
-    double *bufs[] = {ad->ckpt, ad->store, ad->mu, ad->dgf, ad->dh, ad->dpart, ad->apart, ad->grad_rec,
-                      ad->dinj_rec, ad->scratch};
+    double *bufs[] = {ad->ckpt, ad->store, ad->mu, ad->dgf, ad->dh, ad->dpart, ad->acc, ad->grad,
+                      ad->dinj_rec, ad->scratch, ad->zeros};
     for (double *b : bufs)
         if (b) hipFree(b);
     for (hipEvent_t e : ad->ev)
@@ -1404,7 +1395,10 @@ int pic_adjoint_run(pic_adjoint *ad, double sums[2], double d_rec[11], double d_
     HIP_LAUNCH_CHECK();
     hipLaunchKernelGGL(pic::fill_kernel<double>, dim3(256), dim3(256), 0, s->stream, ad->dinj_rec, 2 * T);
     HIP_LAUNCH_CHECK();
-    AdjPending pend;
+    const int64_t nblk = adjoint_blocks(s);
+    hipLaunchKernelGGL(pic::fill_kernel<double>, dim3(256), dim3(256), 0, s->stream, ad->acc,
+                       nblk * 2 * pic::kAdjNV);
+    HIP_LAUNCH_CHECK();
     int cur = 0;
     for (int64_t k = ad->nseg - 1; k >= 0 && rc == PIC_OK; --k) {
         const int64_t s0 = k * K, e = std::min(T, s0 + K);
@@ -1415,21 +1409,19 @@ int pic_adjoint_run(pic_adjoint *ad, double sums[2], double d_rec[11], double d_
         }
         for (int64_t t = s0; t + 1 < e && rc == PIC_OK; ++t) rc = adjoint_recompute_sweep(ad, t, t - s0);
         for (int64_t t = e - 1; t >= s0 && rc == PIC_OK; --t) {
-            rc = adjoint_step(ad, t, t - s0, cur, pend);
+            rc = adjoint_step(ad, t, t - s0, cur);
             cur ^= 1;
         }
     }
     if (rc) return rc;
-    if (pend.out) {
-        hipLaunchKernelGGL(pic::fold_rows_kernel<pic::kAdjNV>, dim3(1), dim3(256), 0, s->stream, pend.part, pend.n,
-                           pic::kAdjStride, pend.out);
-        HIP_LAUNCH_CHECK();
-    }
+    hipLaunchKernelGGL(pic::fold_accumulators_kernel<pic::kAdjNV>, dim3(1), dim3(64), 0, s->stream, ad->acc,
+                       (int)nblk, ad->grad);
+    HIP_LAUNCH_CHECK();
     HIP_TRY(hipEventRecord(ad->ev[2], s->stream));
     // per-step records to the host, summed there in step order
-    std::vector<double> per(T * 4), gr(T * pic::kAdjNV), di(T * 2);
+    std::vector<double> per(T * 4), gr(pic::kAdjNV), di(T * 2);
     HIP_TRY(hipMemcpyAsync(per.data(), s->per_step, T * 4 * 8, hipMemcpyDeviceToHost, s->stream));
-    HIP_TRY(hipMemcpyAsync(gr.data(), ad->grad_rec, T * pic::kAdjNV * 8, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipMemcpyAsync(gr.data(), ad->grad, pic::kAdjNV * 8, hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipMemcpyAsync(di.data(), ad->dinj_rec, T * 2 * 8, hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     HIP_TRY(hipEventElapsedTime(&ad->forward_ms, ad->ev[0], ad->ev[1]));
@@ -1444,11 +1436,7 @@ int pic_adjoint_run(pic_adjoint *ad, double sums[2], double d_rec[11], double d_
         sums[1] = sr;
     }
     if (d_rec)
-        for (int c = 0; c < pic::kAdjNV; ++c) {
-            double acc = 0.0;
-            for (int64_t t = 0; t < T; ++t) acc += gr[pic::kAdjNV * t + c];
-            d_rec[c] = acc;
-        }
+        for (int c = 0; c < pic::kAdjNV; ++c) d_rec[c] = gr[c];
     if (d_inj)
         for (int c = 0; c < 2; ++c) {
             double acc = 0.0;
