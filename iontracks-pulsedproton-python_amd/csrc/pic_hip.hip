@@ -6,7 +6,8 @@
 // computes the deposition density of every in-pulse step of the batch, then
 // one sweep launch per step does deposition-add, Lax-Wendroff update, lateral
 // ring and the per-step reduction into a device (n_steps, 4) record. Nothing
-// is copied back per step.
+// is copied back per step. A batch's launches are collected as data and
+// replayed through graphs of kChain kernel nodes (see submit()).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -15,6 +16,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "pic_hip.h"
@@ -50,6 +52,29 @@ int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 struct SweepShape {
     int tj = 0, chunk = 0, n_windows = 0, wpb = 0, n_kg = 0, n_jt = 0, n_ic = 0, v = 1;
     int64_t blocks() const { return (int64_t)n_kg * n_jt * n_ic; }
+};
+
+// One kernel launch held as data: the kernel, its shape and a copy of its
+// single by-value argument. `kind` classifies it for the timing events
+// (0 sweep, 1 deposition gather, 2 other).
+constexpr size_t kMaxArg = 256;
+struct KLaunch {
+    const void *func = nullptr;
+    dim3 grid, block;
+    int kind = 0;
+    alignas(16) unsigned char arg[kMaxArg];
+
+    template <typename T>
+    static KLaunch make(const void *f, dim3 g, dim3 b, int kind, const T &a) {
+        static_assert(sizeof(T) <= kMaxArg, "kernel argument larger than a launch record");
+        KLaunch l;
+        l.func = f;
+        l.grid = g;
+        l.block = b;
+        l.kind = kind;
+        memcpy(l.arg, &a, sizeof(T));
+        return l;
+    }
 };
 
 }  // namespace
@@ -92,6 +117,9 @@ struct pic_sim {
     std::vector<int> ev_kind;         // per recorded event: 0 sweep, 1 deposit, 2 ring + reduce
     size_t ev_used = 0;
     bool in_run = false;  // events are recorded inside pic_run_steps only
+    bool graphs = true;   // replay a run's launches through graph chains (off while timing)
+    std::vector<KLaunch> *sink = nullptr;  // open segment: launches are collected here
+    std::vector<KLaunch> seg;              // a run's segment buffer, reused
 };
 
 // Many configurations advanced together: per global step one sweep launch
@@ -128,6 +156,8 @@ struct pic_batch {
     std::vector<hipEvent_t> ev_pool;  // pairs around each launch: kind 0 sweep, 1 gather
     std::vector<int> ev_kind;
     size_t ev_used = 0;
+    bool graphs = true;        // replay each segment's launches through graph chains (off while timing)
+    std::vector<KLaunch> seg;  // one segment's launches
 };
 
 // Reverse-mode derivative of one handle's run: checkpoints of the state
@@ -232,36 +262,174 @@ SweepShape compute_sweep_shape(const pic_sim *s) {
     return sh;
 }
 
+// The sweep kernel instantiation for a tiling, deposition flag and wall.
 template <typename S, typename A, int TJ, int V>
-void launch_sweep_tj(const pic::SweepArgs &a, dim3 grid, dim3 block, hipStream_t st, bool dep, bool reflect) {
-    if (reflect) {
-        if (dep)
-            hipLaunchKernelGGL((pic::sweep_kernel<S, A, TJ, true, V, true>), grid, block, 0, st, a);
-        else
-            hipLaunchKernelGGL((pic::sweep_kernel<S, A, TJ, false, V, true>), grid, block, 0, st, a);
-    } else {
-        if (dep)
-            hipLaunchKernelGGL((pic::sweep_kernel<S, A, TJ, true, V, false>), grid, block, 0, st, a);
-        else
-            hipLaunchKernelGGL((pic::sweep_kernel<S, A, TJ, false, V, false>), grid, block, 0, st, a);
-    }
+const void *sweep_fn_tj(bool dep, bool reflect) {
+    if (reflect)
+        return dep ? (const void *)pic::sweep_kernel<S, A, TJ, true, V, true>
+                   : (const void *)pic::sweep_kernel<S, A, TJ, false, V, true>;
+    return dep ? (const void *)pic::sweep_kernel<S, A, TJ, true, V, false>
+               : (const void *)pic::sweep_kernel<S, A, TJ, false, V, false>;
 }
 
 template <typename S, typename A, int V>
-int launch_sweep_typed(const pic::SweepArgs &a, int tj, dim3 grid, dim3 block, hipStream_t st, bool dep,
-                       bool reflect) {
+const void *sweep_fn_typed(int tj, bool dep, bool reflect) {
     switch (tj) {
-        case 1: launch_sweep_tj<S, A, 1, V>(a, grid, block, st, dep, reflect); break;
-        case 2: launch_sweep_tj<S, A, 2, V>(a, grid, block, st, dep, reflect); break;
-        case 4: launch_sweep_tj<S, A, 4, V>(a, grid, block, st, dep, reflect); break;
+        case 1: return sweep_fn_tj<S, A, 1, V>(dep, reflect);
+        case 2: return sweep_fn_tj<S, A, 2, V>(dep, reflect);
+        case 4: return sweep_fn_tj<S, A, 4, V>(dep, reflect);
         case 8:
             if constexpr (V == 2 || sizeof(S) == 8) {
-                return fail(PIC_E_INVALID, "rows_per_tile 8 is for FP32 fields with one value per lane");
+                return nullptr;
             } else {
-                launch_sweep_tj<S, A, 8, V>(a, grid, block, st, dep, reflect);
+                return sweep_fn_tj<S, A, 8, V>(dep, reflect);
             }
+        default: return nullptr;
+    }
+}
+
+// null for a combination that is not compiled (the tiling checks reject it first)
+const void *sweep_fn(int prec, int v, int tj, bool dep, bool reflect) {
+    if (prec == PIC_FP64) return sweep_fn_typed<double, double, 1>(tj, dep, reflect);
+    return v == 2 ? sweep_fn_typed<float, float, 2>(tj, dep, reflect) : sweep_fn_typed<float, float, 1>(tj, dep, reflect);
+}
+
+template <typename S, typename A, int V>
+const void *batch_fn_typed(int tj, bool reflect) {
+    switch (tj) {
+        case 1: return reflect ? (const void *)pic::batch_sweep_kernel<S, A, 1, V, true>
+                               : (const void *)pic::batch_sweep_kernel<S, A, 1, V, false>;
+        case 2: return reflect ? (const void *)pic::batch_sweep_kernel<S, A, 2, V, true>
+                               : (const void *)pic::batch_sweep_kernel<S, A, 2, V, false>;
+        case 4:
+            if constexpr (sizeof(S) == 8) {
+                return nullptr;
+            } else {
+                return reflect ? (const void *)pic::batch_sweep_kernel<S, A, 4, V, true>
+                               : (const void *)pic::batch_sweep_kernel<S, A, 4, V, false>;
+            }
+        default: return nullptr;
+    }
+}
+
+const void *batch_fn(int prec, int v, int tj, bool reflect) {
+    if (prec == PIC_FP64) return batch_fn_typed<double, double, 1>(tj, reflect);
+    return v == 2 ? batch_fn_typed<float, float, 2>(tj, reflect) : batch_fn_typed<float, float, 1>(tj, reflect);
+}
+
+int launch_now(const KLaunch &l, hipStream_t st) {
+    void *argv[1] = {(void *)l.arg};
+    HIP_TRY(hipLaunchKernel(l.func, l.grid, l.block, argv, 0, st));
+    return PIC_OK;
+}
+
+// Graph chains. A chain is a graph of kChain kernel nodes in sequence,
+// instantiated once and then re-parameterised (kernel, grid and argument of
+// every node) for each use, so a run's launches cost one graph launch per
+// kChain kernels instead of one launch each, and the device starts each
+// kernel as the previous one ends. Instances are pooled per device; one is
+// re-parameterised only after the event recorded behind its last launch has
+// completed, so an update never touches a launch still in flight. The pool
+// lives until the process ends.
+constexpr int kChain = 32;
+constexpr int kChainPool = 4;
+
+struct Chain {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipGraphNode_t nodes[kChain] = {};
+    hipEvent_t done = nullptr;
+    uint64_t last_use = 0;
+};
+
+std::mutex g_chain_mu;
+std::vector<std::vector<Chain *>> g_chains;  // [device]
+uint64_t g_chain_uses = 0;
+
+hipKernelNodeParams node_params(const KLaunch &l, void **argv) {
+    hipKernelNodeParams p{};
+    p.func = (void *)l.func;
+    p.gridDim = l.grid;
+    p.blockDim = l.block;
+    p.sharedMemBytes = 0;
+    argv[0] = (void *)l.arg;
+    p.kernelParams = argv;
+    p.extra = nullptr;
+    return p;
+}
+
+int chain_create(const KLaunch *ls, Chain **out) {
+    Chain *c = new Chain();
+    auto cleanup = [&](int code) {
+        if (c->exec) hipGraphExecDestroy(c->exec);
+        if (c->graph) hipGraphDestroy(c->graph);
+        if (c->done) hipEventDestroy(c->done);
+        delete c;
+        return code;
+    };
+    if (hipGraphCreate(&c->graph, 0) != hipSuccess) return cleanup(fail(PIC_E_HIP, "hipGraphCreate failed"));
+    for (int i = 0; i < kChain; ++i) {
+        void *argv[1];
+        const hipKernelNodeParams p = node_params(ls[i], argv);
+        if (hipGraphAddKernelNode(&c->nodes[i], c->graph, i ? &c->nodes[i - 1] : nullptr, i ? 1 : 0, &p) != hipSuccess)
+            return cleanup(fail(PIC_E_HIP, "hipGraphAddKernelNode failed"));
+    }
+    if (hipGraphInstantiate(&c->exec, c->graph, nullptr, nullptr, 0) != hipSuccess)
+        return cleanup(fail(PIC_E_HIP, "hipGraphInstantiate failed"));
+    if (hipEventCreateWithFlags(&c->done, hipEventDisableTiming) != hipSuccess)
+        return cleanup(fail(PIC_E_HIP, "hipEventCreate failed"));
+    *out = c;
+    return PIC_OK;
+}
+
+// run ls[0 .. kChain) on st as one graph launch
+int chain_launch(const KLaunch *ls, int device, hipStream_t st) {
+    std::lock_guard<std::mutex> lock(g_chain_mu);
+    if ((int)g_chains.size() <= device) g_chains.resize(device + 1);
+    std::vector<Chain *> &pool = g_chains[device];
+    Chain *c = nullptr;
+    bool fresh = false;
+    for (Chain *q : pool) {
+        const hipError_t e = hipEventQuery(q->done);
+        if (e == hipSuccess) {
+            c = q;
             break;
-        default: return fail(PIC_E_INVALID, "unsupported rows_per_tile %d (1, 2, 4 or 8)", tj);
+        }
+        if (e != hipErrorNotReady) return fail(PIC_E_HIP, "hipEventQuery failed: %s", hipGetErrorString(e));
+    }
+    if (!c && (int)pool.size() < kChainPool) {
+        int rc = chain_create(ls, &c);
+        if (rc) return rc;
+        pool.push_back(c);
+        fresh = true;
+    }
+    if (!c) {  // all in flight: wait for the least recently launched
+        c = *std::min_element(pool.begin(), pool.end(), [](Chain *x, Chain *y) { return x->last_use < y->last_use; });
+        HIP_TRY(hipEventSynchronize(c->done));
+    }
+    if (!fresh)
+        for (int i = 0; i < kChain; ++i) {
+            void *argv[1];
+            hipKernelNodeParams p = node_params(ls[i], argv);
+            HIP_TRY(hipGraphExecKernelNodeSetParams(c->exec, c->nodes[i], &p));
+        }
+    HIP_TRY(hipGraphLaunch(c->exec, st));
+    HIP_TRY(hipEventRecord(c->done, st));
+    c->last_use = ++g_chain_uses;
+    return PIC_OK;
+}
+
+// Run a segment's launches in order on st: whole chains through graphs, the
+// remainder (fewer than kChain) as ordinary launches.
+int submit(const std::vector<KLaunch> &ls, int device, hipStream_t st) {
+    size_t i = 0;
+    for (; ls.size() - i >= (size_t)kChain; i += kChain) {
+        const int rc = chain_launch(ls.data() + i, device, st);
+        if (rc) return rc;
+    }
+    for (; i < ls.size(); ++i) {
+        const int rc = launch_now(ls[i], st);
+        if (rc) return rc;
     }
     return PIC_OK;
 }
@@ -292,6 +460,27 @@ void record_batch(pic_batch *b, int kind) {
     }
     b->ev_kind[b->ev_used] = kind;
     if (hipEventRecord(b->ev_pool[b->ev_used], b->stream) == hipSuccess) ++b->ev_used;
+}
+
+// A launch of handle s: appended to the open segment, or launched now on st
+// (between a pair of timing events when timing is on).
+int emit(pic_sim *s, const KLaunch &l, hipStream_t st) {
+    if (s->sink) {
+        s->sink->push_back(l);
+        return PIC_OK;
+    }
+    record(s, l.kind);
+    const int rc = launch_now(l, st);
+    if (rc) return rc;
+    record(s, l.kind);
+    return PIC_OK;
+}
+
+const void *checked_sweep_fn(const pic_sim *s, const SweepShape &sh, bool dep) {
+    const void *f = sweep_fn(s->prec, sh.v, sh.tj, dep, s->p.wall == PIC_WALL_REFLECTING);
+    if (!f) fail(PIC_E_INVALID, "no sweep kernel for %d rows per tile with %d values per lane (%s fields)", sh.tj, sh.v,
+                 s->prec == PIC_FP64 ? "FP64" : "FP32");
+    return f;
 }
 
 // Grow the sweep partial buffer to hold nblk blocks (a tiling with more
@@ -364,43 +553,9 @@ int enqueue_sweep(pic_sim *s, int64_t step, bool dep, const double *dens, const 
     if (rc) return rc;
     // one extra block folds the previous step when one is pending
     const dim3 grid((unsigned)(sh.blocks() + (a.prev_out4 ? 1 : 0))), block((unsigned)(64 * sh.wpb));
-    const bool reflect = s->p.wall == PIC_WALL_REFLECTING;
-    record(s, 0);
-    rc = s->prec == PIC_FP64 ? launch_sweep_typed<double, double, 1>(a, sh.tj, grid, block, s->stream, dep, reflect)
-         : sh.v == 2         ? launch_sweep_typed<float, float, 2>(a, sh.tj, grid, block, s->stream, dep, reflect)
-                             : launch_sweep_typed<float, float, 1>(a, sh.tj, grid, block, s->stream, dep, reflect);
-    if (rc) return rc;
-    HIP_LAUNCH_CHECK();
-    record(s, 0);
-    return PIC_OK;
-}
-
-template <typename S, typename A, int V>
-int launch_batch_typed(const pic::SweepArgs *descs, const int *offs, int n_cfg, int tj, dim3 grid, dim3 block,
-                       hipStream_t st, bool reflect) {
-#define PIC_BATCH_LAUNCH(TJ_)                                                                                   \
-    do {                                                                                                        \
-        if (reflect)                                                                                            \
-            hipLaunchKernelGGL((pic::batch_sweep_kernel<S, A, TJ_, V, true>), grid, block, 0, st, descs, offs,  \
-                               n_cfg);                                                                          \
-        else                                                                                                    \
-            hipLaunchKernelGGL((pic::batch_sweep_kernel<S, A, TJ_, V, false>), grid, block, 0, st, descs, offs, \
-                               n_cfg);                                                                          \
-    } while (0)
-    switch (tj) {
-        case 1: PIC_BATCH_LAUNCH(1); break;
-        case 2: PIC_BATCH_LAUNCH(2); break;
-        case 4:
-            if constexpr (sizeof(S) == 8) {
-                return fail(PIC_E_INVALID, "batched FP64 sweeps take 1 or 2 rows per tile");
-            } else {
-                PIC_BATCH_LAUNCH(4);
-            }
-            break;
-        default: return fail(PIC_E_INVALID, "unsupported batched rows_per_tile %d", tj);
-    }
-#undef PIC_BATCH_LAUNCH
-    return PIC_OK;
+    const void *f = checked_sweep_fn(s, sh, dep);
+    if (!f) return PIC_E_INVALID;
+    return emit(s, KLaunch::make(f, grid, block, 0, a), s->stream);
 }
 
 // fold the pending step's partials into its record (end of a run of steps)
@@ -448,12 +603,9 @@ pic::GatherArgs make_gather_args(const pic_sim *s, const double *xs, const doubl
 int enqueue_gather(pic_sim *s, const double *xs, const double *ys, const int64_t *offsets, int64_t first,
                    int64_t count, hipStream_t st = nullptr) {
     const pic::GatherArgs g = make_gather_args(s, xs, ys, offsets, first);
-    record(s, 1);
-    hipLaunchKernelGGL(pic::gather_kernel, dim3((unsigned)gather_blocks(s), (unsigned)count), dim3(256), 0,
-                       st ? st : s->stream, g);
-    HIP_LAUNCH_CHECK();
-    record(s, 1);
-    return PIC_OK;
+    return emit(s, KLaunch::make((const void *)pic::gather_kernel, dim3((unsigned)gather_blocks(s), (unsigned)count),
+                                 dim3(256), 1, g),
+                st ? st : s->stream);
 }
 
 int enqueue_deposit_reduce(pic_sim *s, double *out4) {
@@ -605,10 +757,9 @@ int adjoint_recompute_sweep(pic_adjoint *ad, int64_t t, int64_t src) {
     a.neg_out = in + 3 * s->n_vox;
     a.prev_out4 = nullptr;
     const dim3 grid((unsigned)sh.blocks()), block((unsigned)(64 * sh.wpb));
-    rc = launch_sweep_typed<double, double, 1>(a, sh.tj, grid, block, s->stream, dep, s->p.wall == PIC_WALL_REFLECTING);
-    if (rc) return rc;
-    HIP_LAUNCH_CHECK();
-    return PIC_OK;
+    const void *f = checked_sweep_fn(s, sh, dep);
+    if (!f) return PIC_E_INVALID;
+    return launch_now(KLaunch::make(f, grid, block, 0, a), s->stream);
 }
 
 int adjoint_deriv_gather(pic_adjoint *ad, int64_t first, int64_t count) {
@@ -833,8 +984,15 @@ int pic_run_steps(pic_sim *s, int64_t first, int64_t count) {
     const int n_tiles = (int)gather_blocks(s);
     const int64_t xy = s->p.no_xy * s->p.no_xy;
     const int64_t slots = s->batch + 1;
+    // one segment per gather batch: its gather and sweeps are collected, then
+    // replayed through graph chains (launched one by one while timing)
+    const bool chained = s->graphs && !s->timing;
     for (int64_t b0 = first; b0 < first + count && rc == PIC_OK; b0 += s->batch) {
         const int64_t b1 = std::min(first + count, b0 + s->batch);
+        if (chained) {
+            s->seg.clear();
+            s->sink = &s->seg;
+        }
         if (s->offsets[b1] > s->offsets[b0]) rc = enqueue_gather(s, s->xs, s->ys, s->offsets_d, b0, b1 - b0);
         for (int64_t step = b0; step < b1 && rc == PIC_OK; ++step) {
             const bool dep = s->offsets[step + 1] > s->offsets[step];
@@ -843,6 +1001,8 @@ int pic_run_steps(pic_sim *s, int64_t first, int64_t count) {
                                s->per_step + 4 * step);
             if (!rc) swap_buffers(s);
         }
+        s->sink = nullptr;
+        if (chained && !rc) rc = submit(s->seg, s->device, s->stream);
     }
     if (!rc) rc = flush_pending(s);
     s->in_run = false;
@@ -1012,6 +1172,12 @@ int pic_set_tiling(pic_sim *s, int rows_per_tile, int planes_per_chunk, int valu
 int pic_enable_timing(pic_sim *s, int enable) {
     if (!s) return fail(PIC_E_INVALID, "sim is NULL");
     s->timing = enable != 0;
+    return PIC_OK;
+}
+
+int pic_set_graphs(pic_sim *s, int enable) {
+    if (!s) return fail(PIC_E_INVALID, "sim is NULL");
+    s->graphs = enable != 0;
     return PIC_OK;
 }
 
@@ -1243,26 +1409,31 @@ int pic_batch_run(pic_batch *b, int64_t first, int64_t count) {
         }
         HIP_TRY(hipEventRecord(b->slot_ev[k], b->stream));
         b->slot_busy[k] = true;
-        if (ng > 0) {
-            record_batch(b, 1);
-            hipLaunchKernelGGL(pic::batch_gather_kernel, dim3((unsigned)goffs[ng]), dim3(256), 0, b->stream,
-                               b->gdesc_d[k], b->goffs_d[k], ng);
-            if (hipGetLastError() != hipSuccess) rc = fail(PIC_E_HIP, "batched gather launch failed");
-            record_batch(b, 1);
-        }
+        // the segment's launches: the gather, then one sweep per step
+        b->seg.clear();
+        if (ng > 0)
+            b->seg.push_back(KLaunch::make((const void *)pic::batch_gather_kernel, dim3((unsigned)goffs[ng]), dim3(256), 1,
+                                           pic::BatchTable<pic::GatherArgs>{b->gdesc_d[k], b->goffs_d[k], ng}));
         for (int64_t t = s0; t < s1 && rc == PIC_OK; ++t) {
             const int na = active[t - s0];
             const int *o = offs + (t - s0) * (n + 1);
             const dim3 grid((unsigned)(o[na] + na));  // na fold blocks, then the sweep blocks
-            const pic::SweepArgs *d = b->descs_d[k] + (t - s0) * n;
-            const int *od = b->offs_d[k] + (t - s0) * (n + 1);
+            const pic::BatchTable<pic::SweepArgs> table{b->descs_d[k] + (t - s0) * n, b->offs_d[k] + (t - s0) * (n + 1),
+                                                        na};
             const int tj = b->cands[cand_of[t - s0]].first;
-            record_batch(b, 0);
-            rc = b->prec == PIC_FP64 ? launch_batch_typed<double, double, 1>(d, od, na, tj, grid, block, b->stream, b->reflect)
-                 : b->v == 2         ? launch_batch_typed<float, float, 2>(d, od, na, tj, grid, block, b->stream, b->reflect)
-                                     : launch_batch_typed<float, float, 1>(d, od, na, tj, grid, block, b->stream, b->reflect);
-            if (!rc && hipGetLastError() != hipSuccess) rc = fail(PIC_E_HIP, "batched sweep launch failed");
-            record_batch(b, 0);
+            const void *f = batch_fn(b->prec, b->v, tj, b->reflect);
+            if (!f) rc = fail(PIC_E_INVALID, "no batched sweep kernel for %d rows per tile", tj);
+            else b->seg.push_back(KLaunch::make(f, grid, block, 0, table));
+        }
+        if (rc) break;
+        if (b->graphs && !b->timing) {
+            rc = submit(b->seg, b->sims[0]->device, b->stream);
+        } else {
+            for (const KLaunch &l : b->seg) {
+                record_batch(b, l.kind);
+                if ((rc = launch_now(l, b->stream))) break;
+                record_batch(b, l.kind);
+            }
         }
     }
     for (pic_sim *s : b->sims)
@@ -1277,6 +1448,12 @@ int pic_batch_run(pic_batch *b, int64_t first, int64_t count) {
 int pic_batch_enable_timing(pic_batch *b, int enable) {
     if (!b) return fail(PIC_E_INVALID, "batch is NULL");
     b->timing = enable != 0;
+    return PIC_OK;
+}
+
+int pic_batch_set_graphs(pic_batch *b, int enable) {
+    if (!b) return fail(PIC_E_INVALID, "batch is NULL");
+    b->graphs = enable != 0;
     return PIC_OK;
 }
 

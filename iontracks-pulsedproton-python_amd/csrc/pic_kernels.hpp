@@ -552,6 +552,17 @@ __global__ void __launch_bounds__(512) sweep_kernel(SweepArgs a) {
     sweep_body<S, A, TJ, V, REFLECT>(a, xcd_logical_block(blockIdx.x - nf, gridDim.x - nf), DEP);
 }
 
+// The one argument of a batched launch: per-configuration descriptors, the
+// first block of each configuration (offs[c]; offs[n_cfg] = all blocks) and
+// the number of configurations. A single by-value argument lets a launch be
+// held as data and replayed as a graph node.
+template <typename D>
+struct BatchTable {
+    const D *descs;
+    const int *offs;
+    int n_cfg;
+};
+
 // Many configurations, one step each, in one launch. Blocks [0, n_cfg) fold
 // configuration b's previous step (if pending); sweep block g (after the XCD
 // remap) belongs to configuration c with offs[c] <= g < offs[c+1] and runs it
@@ -562,8 +573,10 @@ __global__ void __launch_bounds__(512) sweep_kernel(SweepArgs a) {
 // dens at zeros, and x + 0.0 == x. Configurations are ordered so the active
 // ones are a prefix; the grid covers exactly those.
 template <typename S, typename A, int TJ, int V, bool REFLECT>
-__global__ void __launch_bounds__(512) batch_sweep_kernel(const SweepArgs *__restrict__ descs,
-                                                          const int *__restrict__ offs, int n_cfg) {
+__global__ void __launch_bounds__(512) batch_sweep_kernel(BatchTable<SweepArgs> t) {
+    const SweepArgs *__restrict__ descs = t.descs;
+    const int *__restrict__ offs = t.offs;
+    const int n_cfg = t.n_cfg;
     if ((int)blockIdx.x < n_cfg) {
         const SweepArgs a = descs[blockIdx.x];
         if (a.prev_out4) fold_previous(a);
@@ -735,8 +748,10 @@ __global__ void __launch_bounds__(256) gather_kernel(GatherArgs a) { gather_body
 // configuration c with offs[c] <= g < offs[c+1] (c found by one ballot over
 // the offsets) and runs tile (g - offs[c]) % tiles of step (g - offs[c]) /
 // tiles of that configuration's arguments descs[c].
-__global__ void __launch_bounds__(256) batch_gather_kernel(const GatherArgs *__restrict__ descs,
-                                                           const int *__restrict__ offs, int n_cfg) {
+__global__ void __launch_bounds__(256) batch_gather_kernel(BatchTable<GatherArgs> t) {
+    const GatherArgs *__restrict__ descs = t.descs;
+    const int *__restrict__ offs = t.offs;
+    const int n_cfg = t.n_cfg;
     const int g = blockIdx.x;
     const int lane = threadIdx.x & 63;
     int c = 0;

@@ -48,6 +48,7 @@ EXPORTED = (
     "pic_set_tiling",
     "pic_enable_timing",
     "pic_get_timing",
+    "pic_set_graphs",
     "pic_batch_create",
     "pic_batch_destroy",
     "pic_batch_set_tiling",
@@ -56,6 +57,7 @@ EXPORTED = (
     "pic_batch_synchronize",
     "pic_batch_enable_timing",
     "pic_batch_get_timing",
+    "pic_batch_set_graphs",
     "pic_adjoint_create",
     "pic_adjoint_destroy",
     "pic_adjoint_run",
@@ -133,6 +135,7 @@ def open_library():
         "pic_set_tiling": [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int],
         "pic_enable_timing": [_VP, ctypes.c_int],
         "pic_get_timing": [_VP, _DP, _IP, _DP, _IP, _DP],
+        "pic_set_graphs": [_VP, ctypes.c_int],
         "pic_batch_create": [ctypes.POINTER(_VP), ctypes.c_int, ctypes.POINTER(_VP)],
         "pic_batch_destroy": [_VP],
         "pic_batch_set_tiling": [_VP, ctypes.c_int, ctypes.c_int],
@@ -141,6 +144,7 @@ def open_library():
         "pic_batch_synchronize": [_VP],
         "pic_batch_enable_timing": [_VP, ctypes.c_int],
         "pic_batch_get_timing": [_VP, _DP, _IP, _DP, _IP],
+        "pic_batch_set_graphs": [_VP, ctypes.c_int],
         "pic_adjoint_create": [_VP, ctypes.c_int64, ctypes.POINTER(_VP)],
         "pic_adjoint_destroy": [_VP],
         "pic_adjoint_run": [_VP, _DP, _DP, _DP],

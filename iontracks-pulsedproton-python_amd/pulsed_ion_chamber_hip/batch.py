@@ -71,6 +71,10 @@ class HipBatch:
     def enable_timing(self, on: bool = True) -> None:
         _lib.check(self.L.pic_batch_enable_timing(self.handle, 1 if on else 0), "pic_batch_enable_timing")
 
+    def set_graphs(self, on: bool = True) -> None:
+        """Replay each segment's launches through graphs (the default) or launch them one by one."""
+        _lib.check(self.L.pic_batch_set_graphs(self.handle, 1 if on else 0), "pic_batch_set_graphs")
+
     def timing(self) -> dict:
         sweep_ms = ctypes.c_double()
         gather_ms = ctypes.c_double()

@@ -123,6 +123,10 @@ class HipSimulation:
     def enable_timing(self, on: bool = True) -> None:
         _lib.check(self.L.pic_enable_timing(self.handle, 1 if on else 0), "pic_enable_timing")
 
+    def set_graphs(self, on: bool = True) -> None:
+        """Replay run_steps' launches through graphs (the default) or launch them one by one."""
+        _lib.check(self.L.pic_set_graphs(self.handle, 1 if on else 0), "pic_set_graphs")
+
     def timing(self) -> dict:
         sweep_ms = ctypes.c_double()
         dep_ms = ctypes.c_double()
