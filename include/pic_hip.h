@@ -132,11 +132,15 @@ int pic_enable_timing(pic_sim *sim, int enable);
 int pic_get_timing(pic_sim *sim, double *sweep_ms, int64_t *sweep_launches, double *deposit_ms,
                    int64_t *deposit_launches, double *other_ms);
 
-/* Graph replay of pic_run_steps (on by default): each gather batch's
+/* Graph replay of pic_run_steps (off by default): each gather batch's
  * launches are collected and run through graphs of 32 kernel nodes,
  * re-parameterised per use, instead of one launch at a time; the kernels and
- * their results are the same. Timing (pic_enable_timing) launches one at a
- * time regardless. Replaces the per-step launch loop of solver_cuda.py:791-865. */
+ * their results are the same. It frees the host thread (about 1 us of host
+ * time per kernel instead of 4); on MI355X under ROCm 7 the device runs the
+ * replayed kernels slightly slower (+5% on small-grid runs, whose kernels take
+ * ~7 us), so it is not the default. Timing (pic_enable_timing) launches one
+ * at a time regardless. Replaces the per-step launch loop of
+ * solver_cuda.py:791-865. */
 int pic_set_graphs(pic_sim *sim, int enable);
 
 /* Batched runs: many handles (a voltage / dose-rate / seed sweep with uneven
@@ -169,7 +173,7 @@ int pic_batch_synchronize(pic_batch *batch);
 int pic_batch_enable_timing(pic_batch *batch, int enable);
 int pic_batch_get_timing(pic_batch *batch, double *sweep_ms, int64_t *sweep_launches, double *gather_ms,
                          int64_t *gather_launches);
-/* Graph replay of pic_batch_run's segments (on by default), as pic_set_graphs. */
+/* Graph replay of pic_batch_run's segments (off by default), as pic_set_graphs. */
 int pic_batch_set_graphs(pic_batch *batch, int enable);
 
 /* Reverse-mode derivative of one FP64 handle's run (no counterpart in the

@@ -6,8 +6,9 @@
 // computes the deposition density of every in-pulse step of the batch, then
 // one sweep launch per step does deposition-add, Lax-Wendroff update, lateral
 // ring and the per-step reduction into a device (n_steps, 4) record. Nothing
-// is copied back per step. A batch's launches are collected as data and
-// replayed through graphs of kChain kernel nodes (see submit()).
+// is copied back per step. With graph replay on, a segment's launches are
+// collected as data and replayed through graphs of kChain kernel nodes (see
+// submit()).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -117,7 +118,7 @@ struct pic_sim {
     std::vector<int> ev_kind;         // per recorded event: 0 sweep, 1 deposit, 2 ring + reduce
     size_t ev_used = 0;
     bool in_run = false;  // events are recorded inside pic_run_steps only
-    bool graphs = true;   // replay a run's launches through graph chains (off while timing)
+    bool graphs = false;  // replay a run's launches through graph chains (never while timing)
     std::vector<KLaunch> *sink = nullptr;  // open segment: launches are collected here
     std::vector<KLaunch> seg;              // a run's segment buffer, reused
 };
@@ -156,7 +157,7 @@ struct pic_batch {
     std::vector<hipEvent_t> ev_pool;  // pairs around each launch: kind 0 sweep, 1 gather
     std::vector<int> ev_kind;
     size_t ev_used = 0;
-    bool graphs = true;        // replay each segment's launches through graph chains (off while timing)
+    bool graphs = false;       // replay each segment's launches through graph chains (never while timing)
     std::vector<KLaunch> seg;  // one segment's launches
 };
 
@@ -376,7 +377,9 @@ int chain_create(const KLaunch *ls, Chain **out) {
     }
     if (hipGraphInstantiate(&c->exec, c->graph, nullptr, nullptr, 0) != hipSuccess)
         return cleanup(fail(PIC_E_HIP, "hipGraphInstantiate failed"));
-    if (hipEventCreateWithFlags(&c->done, hipEventDisableTiming) != hipSuccess)
+    // device-scope release: the event only marks the launch as consumed; a
+    // system-scope one would write the L2 back to memory behind every chain
+    if (hipEventCreateWithFlags(&c->done, hipEventDisableTiming | hipEventReleaseToDevice) != hipSuccess)
         return cleanup(fail(PIC_E_HIP, "hipEventCreate failed"));
     *out = c;
     return PIC_OK;
@@ -432,6 +435,16 @@ int submit(const std::vector<KLaunch> &ls, int device, hipStream_t st) {
         if (rc) return rc;
     }
     return PIC_OK;
+}
+
+// a batched launch's argument
+template <typename D>
+pic::BatchTable<D> batch_table(const D *descs, const int *offs, int n_cfg) {
+    pic::BatchTable<D> t{};
+    t.descs = descs;
+    t.offs = offs;
+    t.n_cfg = n_cfg;
+    return t;
 }
 
 // With timing on, every kernel group is bracketed by a pair of events on the
@@ -1413,13 +1426,12 @@ int pic_batch_run(pic_batch *b, int64_t first, int64_t count) {
         b->seg.clear();
         if (ng > 0)
             b->seg.push_back(KLaunch::make((const void *)pic::batch_gather_kernel, dim3((unsigned)goffs[ng]), dim3(256), 1,
-                                           pic::BatchTable<pic::GatherArgs>{b->gdesc_d[k], b->goffs_d[k], ng}));
+                                           batch_table(b->gdesc_d[k], b->goffs_d[k], ng)));
         for (int64_t t = s0; t < s1 && rc == PIC_OK; ++t) {
             const int na = active[t - s0];
             const int *o = offs + (t - s0) * (n + 1);
             const dim3 grid((unsigned)(o[na] + na));  // na fold blocks, then the sweep blocks
-            const pic::BatchTable<pic::SweepArgs> table{b->descs_d[k] + (t - s0) * n, b->offs_d[k] + (t - s0) * (n + 1),
-                                                        na};
+            const auto table = batch_table(b->descs_d[k] + (t - s0) * n, b->offs_d[k] + (t - s0) * (n + 1), na);
             const int tj = b->cands[cand_of[t - s0]].first;
             const void *f = batch_fn(b->prec, b->v, tj, b->reflect);
             if (!f) rc = fail(PIC_E_INVALID, "no batched sweep kernel for %d rows per tile", tj);

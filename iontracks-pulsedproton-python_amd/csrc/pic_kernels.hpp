@@ -555,13 +555,30 @@ __global__ void __launch_bounds__(512) sweep_kernel(SweepArgs a) {
 // The one argument of a batched launch: per-configuration descriptors, the
 // first block of each configuration (offs[c]; offs[n_cfg] = all blocks) and
 // the number of configurations. A single by-value argument lets a launch be
-// held as data and replayed as a graph node.
+// held as data and replayed as a graph node. (Carrying the offsets in the
+// argument itself measured slower on MI355X than one vector load of this
+// table: kernel arguments are fetched by dependent scalar loads.)
 template <typename D>
 struct BatchTable {
     const D *descs;
     const int *offs;
     int n_cfg;
 };
+
+// The configuration c of batched block g (offs[c] <= g < offs[c+1]) and its
+// first block offs[c]: c = the number of offs[1..n_cfg-1] <= g, one load per
+// lane instead of a chain of dependent scalar loads.
+template <typename D>
+__device__ __forceinline__ void find_config(const BatchTable<D> &t, int g, int &c, int &first) {
+    const int lane = threadIdx.x & 63;
+    int cc = 0;
+    for (int b = 1; b < t.n_cfg; b += kWave) {
+        const int q = b + lane;
+        cc += __popcll(__ballot(q < t.n_cfg && t.offs[q] <= g));
+    }
+    c = __builtin_amdgcn_readfirstlane(cc);
+    first = t.offs[c];
+}
 
 // Many configurations, one step each, in one launch. Blocks [0, n_cfg) fold
 // configuration b's previous step (if pending); sweep block g (after the XCD
@@ -575,7 +592,6 @@ struct BatchTable {
 template <typename S, typename A, int TJ, int V, bool REFLECT>
 __global__ void __launch_bounds__(512) batch_sweep_kernel(BatchTable<SweepArgs> t) {
     const SweepArgs *__restrict__ descs = t.descs;
-    const int *__restrict__ offs = t.offs;
     const int n_cfg = t.n_cfg;
     if ((int)blockIdx.x < n_cfg) {
         const SweepArgs a = descs[blockIdx.x];
@@ -583,17 +599,10 @@ __global__ void __launch_bounds__(512) batch_sweep_kernel(BatchTable<SweepArgs> 
         return;
     }
     const int g = xcd_logical_block(blockIdx.x - n_cfg, gridDim.x - n_cfg);
-    // c = number of offs[1..n_cfg-1] <= g: one load per lane instead of a
-    // chain of dependent scalar loads
-    const int lane = threadIdx.x & 63;
-    int c = 0;
-    for (int base = 1; base < n_cfg; base += kWave) {
-        const int q = base + lane;
-        c += __popcll(__ballot(q < n_cfg && offs[q] <= g));
-    }
-    c = __builtin_amdgcn_readfirstlane(c);
+    int c, first;
+    find_config(t, g, c, first);
     const SweepArgs a = descs[c];
-    sweep_body<S, A, TJ, V, REFLECT>(a, g - offs[c], true);
+    sweep_body<S, A, TJ, V, REFLECT>(a, g - first, true);
 }
 
 struct GatherArgs {
@@ -745,24 +754,14 @@ __device__ __forceinline__ void gather_body(const GatherArgs &a, const int tile,
 __global__ void __launch_bounds__(256) gather_kernel(GatherArgs a) { gather_body<false>(a, blockIdx.x, blockIdx.y); }
 
 // The gathers of many configurations in one launch: block g belongs to
-// configuration c with offs[c] <= g < offs[c+1] (c found by one ballot over
-// the offsets) and runs tile (g - offs[c]) % tiles of step (g - offs[c]) /
-// tiles of that configuration's arguments descs[c].
+// configuration c with offs[c] <= g < offs[c+1] and runs tile
+// (g - offs[c]) % tiles of step (g - offs[c]) / tiles of that
+// configuration's arguments descs[c].
 __global__ void __launch_bounds__(256) batch_gather_kernel(BatchTable<GatherArgs> t) {
-    const GatherArgs *__restrict__ descs = t.descs;
-    const int *__restrict__ offs = t.offs;
-    const int n_cfg = t.n_cfg;
-    const int g = blockIdx.x;
-    const int lane = threadIdx.x & 63;
-    int c = 0;
-    for (int base = 1; base < n_cfg; base += kWave) {
-        const int q = base + lane;
-        c += __popcll(__ballot(q < n_cfg && offs[q] <= g));
-    }
-    // every wave of the block computed the same c; make it uniform
-    c = __builtin_amdgcn_readfirstlane(c);
-    const GatherArgs a = descs[c];
-    const int local = g - offs[c];
+    int c, first;
+    find_config(t, (int)blockIdx.x, c, first);
+    const GatherArgs a = t.descs[c];
+    const int local = (int)blockIdx.x - first;
     const int tiles = a.ntj * a.ntj;
     gather_body<false>(a, local % tiles, local / tiles);
 }

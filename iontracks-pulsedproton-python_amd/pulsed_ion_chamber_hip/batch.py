@@ -72,7 +72,7 @@ class HipBatch:
         _lib.check(self.L.pic_batch_enable_timing(self.handle, 1 if on else 0), "pic_batch_enable_timing")
 
     def set_graphs(self, on: bool = True) -> None:
-        """Replay each segment's launches through graphs (the default) or launch them one by one."""
+        """Replay each segment's launches through graphs, or launch them one by one (the default)."""
         _lib.check(self.L.pic_batch_set_graphs(self.handle, 1 if on else 0), "pic_batch_set_graphs")
 
     def timing(self) -> dict:

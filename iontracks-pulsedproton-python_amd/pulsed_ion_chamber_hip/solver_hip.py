@@ -124,7 +124,7 @@ class HipSimulation:
         _lib.check(self.L.pic_enable_timing(self.handle, 1 if on else 0), "pic_enable_timing")
 
     def set_graphs(self, on: bool = True) -> None:
-        """Replay run_steps' launches through graphs (the default) or launch them one by one."""
+        """Replay run_steps' launches through graphs, or launch them one by one (the default)."""
         _lib.check(self.L.pic_set_graphs(self.handle, 1 if on else 0), "pic_set_graphs")
 
     def timing(self) -> dict:

@@ -1,6 +1,6 @@
 """Graph replay of a run's launches must not change a single bit.
 
-By default a run's launches (each gather batch's deposition gather and its
+With graph replay on, a run's launches (each gather batch's deposition gather and its
 per-step sweeps) are collected and replayed through graphs of kernel nodes
 that are re-parameterised for every use -- kernel, grid and argument -- so
 one node carries a depositing sweep in one use and a plain sweep (another
