@@ -87,26 +87,33 @@ struct AdjointArgs {
     double *op, *on;         // mu_t
     double *acc;             // [nblk][2 * kAdjNV]: each block's running (sum, compensation) over the steps
     int no_xy, nzb, nze, no_z, pitch;
-    int n_kb, n_jb;  // 64-wide k blocks and 4-row j blocks per i plane
+    int n_kb, n_jb;  // 64-wide k blocks and 4-row j blocks per i chunk
+    int chunk;       // i planes each block marches through
     double mid, rsq;
     double p_lat, p_zm, p_zp, p_cen, n_lat, n_zm, n_zp, n_cen, adt;
 };
 
-// One thread per voxel (i, j, k) of the whole grid, ring included: 64 k by 4
-// j rows per block, one i plane per block row. Neighbours come straight from
-// memory (the grids this is for -- tens of columns a side -- stay in cache),
-// all loads issued before the first use. Only the run's totals of the
-// sensitivities are wanted, so each block adds its step sums to its own
-// compensated running sum (the same block, the same slot, in step order:
-// deterministic and with no dependence between blocks); the blocks' sums are
-// folded once, after the last step, by fold_accumulators_kernel.
+// One thread per (j, k) column of the whole grid, ring included (64 k by 4 j
+// per block), marching through a chunk of i planes, as the forward sweep
+// does: the thread's own column at planes i-1, i, i+1 (mu_{t+1} and X_t)
+// stays in registers and moves down the march; the j and k neighbours of
+// plane i are loaded a plane ahead, as is the own column of plane i+2, so a
+// step waits only for loads issued one step earlier. Every load is
+// unconditional (indices clamped into the grid) and every contribution is
+// selected by its predicate (a skipped term is an added +0.0), so each
+// voxel's value is the one of the per-voxel form term for term.
+// Only the run's totals of the sensitivities are wanted: each thread sums
+// its chunk's voxels, each block adds its step sum to its own compensated
+// running sum (the same block, the same slot, in step order: deterministic
+// and with no dependence between blocks), and fold_accumulators_kernel folds
+// the blocks' sums once, after the last step.
 template <bool REFLECT>
 __global__ void __launch_bounds__(256) adjoint_kernel(AdjointArgs a) {
     const int b = blockIdx.x;
     const int kb = b % a.n_kb;
     const int rest = b / a.n_kb;
     const int jb = rest % a.n_jb;
-    const int i = rest / a.n_jb;
+    const int ic = rest / a.n_jb;
     const int k = kb * kWave + (threadIdx.x & 63);
     const int j = jb * 4 + (threadIdx.x >> 6);
     const int n = a.no_xy, nzb = a.nzb;
@@ -115,124 +122,187 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjointArgs a) {
     for (int c = 0; c < kAdjNV; ++c) g[c] = 0.0;
 
     if (k < nzb && j < n) {
-        // Every load below is unconditional (indices clamped into the grid) and
-        // every contribution is selected by its predicate, so the compiler can
-        // keep all of a thread's loads in flight at once; a skipped term is an
-        // added +0.0, which leaves each sum bit-identical.
-        auto at = [&](int ii, int jj, int kk) -> int64_t {
-            ii = min(max(ii, 0), n - 1);
-            jj = min(max(jj, 0), n - 1);
-            kk = min(max(kk, 0), nzb - 1);
-            return ((int64_t)ii * n + jj) * a.pitch + kk;
-        };
+        const int i0 = ic * a.chunk, i1 = min(i0 + a.chunk, n);
+        const int64_t rowp = a.pitch, plane = (int64_t)n * a.pitch;
+        auto clampi = [](int x, int lo, int hi) { return min(max(x, lo), hi); };
         auto in_gap = [&](int kk) { return kk >= a.nze && kk < a.nze + a.no_z; };
         auto interior = [&](int ii, int jj, int kk) {
             return ii >= 1 && ii <= n - 2 && jj >= 1 && jj <= n - 2 && kk >= 1 && kk <= nzb - 2;
         };
-        // X_t = S_t + d_t, the swept input (a reflecting ring reads its mirror
-        // column and adds its own column's deposition, as the sweep does)
-        auto X = [&](const double *S, int ii, int jj, int kk) -> double {
-            const int si = REFLECT ? min(max(ii, 1), n - 2) : ii;
-            const int sj = REFLECT ? min(max(jj, 1), n - 2) : jj;
-            const double v = S[at(si, sj, kk)];
-            const double d = a.dens[(int64_t)min(max(ii, 0), n - 1) * n + min(max(jj, 0), n - 1)];
-            return v + (in_gap(kk) ? d : 0.0);
+        // the stored column of (ii, jj) (clamped into the grid) and the column
+        // a swept input is read from: under a reflecting wall a ring column
+        // reads its mirror, as the sweep does
+        auto acol = [&](int ii, int jj) -> int64_t {
+            return (int64_t)clampi(ii, 0, n - 1) * plane + (int64_t)clampi(jj, 0, n - 1) * rowp;
         };
-        // mu_{t+1} at interior (ii, jj, kk); under a reflecting wall it is
-        // stored with the ring already folded in (below)
-        auto A = [&](const double *M, int ii, int jj, int kk) -> double { return M[at(ii, jj, kk)]; };
-        const bool u_int = interior(i, j, k);
-        double op = 0.0, on = 0.0;
-        if (!REFLECT) {  // carried ring and k end planes
-            const double cp = a.mp[at(i, j, k)], cn = a.mn[at(i, j, k)];
-            op = u_int ? 0.0 : cp;
-            on = u_int ? 0.0 : cn;
-        }
-        // transposed stencil: v = u + offset must be interior
-        const bool zp1 = interior(i, j, k + 1), zm1 = interior(i, j, k - 1);
-        const bool jp1 = interior(i, j + 1, k), jm1 = interior(i, j - 1, k);
-        const bool ip1 = interior(i + 1, j, k), im1 = interior(i - 1, j, k);
-        const double ap_zp = A(a.mp, i, j, k + 1), ap_zm = A(a.mp, i, j, k - 1);
-        const double ap_jp = A(a.mp, i, j + 1, k), ap_jm = A(a.mp, i, j - 1, k);
-        const double ap_ip = A(a.mp, i + 1, j, k), ap_im = A(a.mp, i - 1, j, k);
-        const double an_zp = A(a.mn, i, j, k + 1), an_zm = A(a.mn, i, j, k - 1);
-        const double an_jp = A(a.mn, i, j + 1, k), an_jm = A(a.mn, i, j - 1, k);
-        const double an_ip = A(a.mn, i + 1, j, k), an_im = A(a.mn, i - 1, j, k);
-        const double ap = A(a.mp, i, j, k), an = A(a.mn, i, j, k);
-        const double xp = X(a.sp, i, j, k), xn = X(a.sn, i, j, k);
-        const double xp_jm = X(a.sp, i, j - 1, k), xp_jp = X(a.sp, i, j + 1, k);
-        const double xp_im = X(a.sp, i - 1, j, k), xp_ip = X(a.sp, i + 1, j, k);
-        const double xp_km = X(a.sp, i, j, k - 1), xp_kp = X(a.sp, i, j, k + 1);
-        const double xn_jm = X(a.sn, i, j - 1, k), xn_jp = X(a.sn, i, j + 1, k);
-        const double xn_im = X(a.sn, i - 1, j, k), xn_ip = X(a.sn, i + 1, j, k);
-        const double xn_km = X(a.sn, i, j, k - 1), xn_kp = X(a.sn, i, j, k + 1);
-        const int64_t col = (int64_t)i * n + j;
-        const double dgf = a.dgf[col], dh = a.dh[col];
-        op += zp1 ? a.p_zm * ap_zp : 0.0;
-        op += zm1 ? a.p_zp * ap_zm : 0.0;
-        op += jp1 ? a.p_lat * ap_jp : 0.0;
-        op += jm1 ? a.p_lat * ap_jm : 0.0;
-        op += ip1 ? a.p_lat * ap_ip : 0.0;
-        op += im1 ? a.p_lat * ap_im : 0.0;
-        on += zm1 ? a.n_zm * an_zm : 0.0;
-        on += zp1 ? a.n_zp * an_zp : 0.0;
-        on += jp1 ? a.n_lat * an_jp : 0.0;
-        on += jm1 ? a.n_lat * an_jm : 0.0;
-        on += ip1 ? a.n_lat * an_ip : 0.0;
-        on += im1 ? a.n_lat * an_im : 0.0;
-        if (u_int) {
-            const double s = ap + an;
-            op += a.p_cen * ap;
-            on += a.n_cen * an;
-            op -= a.adt * xn * s;
-            on -= a.adt * xp * s;
-            const double di = (double)i - a.mid, dj = (double)j - a.mid;
-            const bool scored = in_gap(k) && di * di + dj * dj < a.rsq;
-            op += scored ? a.adt * xn : 0.0;
-            on += scored ? a.adt * xp : 0.0;
-            // sensitivities of H (and r) at this interior voxel
-            g[0] = ap * (((xp_jm + xp_jp) + xp_im) + xp_ip);
-            g[1] = ap * xp_km;
-            g[2] = ap * xp_kp;
-            g[3] = ap * xp;
-            g[4] = an * (((xn_jm + xn_jp) + xn_im) + xn_ip);
-            g[5] = an * xn_kp;
-            g[6] = an * xn_km;
-            g[7] = an * xn;
-            const double pn = xp * xn;
-            g[8] = (scored ? pn : 0.0) - s * pn;
-        }
-        if (in_gap(k) && (!REFLECT || u_int)) {  // (a reflecting ring's terms are taken by its partner)
-            g[9] = (op + on) * dgf;
-            g[10] = (op + on) * dh;
-        }
-        if constexpr (REFLECT) {
-            // A ring voxel next to this one reads this column (the ring is a
-            // copy of its neighbour), and its only interior consumer is this
-            // voxel, through the lateral weight: its adjoint is lat * mu(u).
-            // Its deposition term is taken here, with the ring column's
-            // dd/dgf and dd/dh2b2, and its adjoint is folded into this voxel's
-            // -- the transposed copy -- so the ring itself is never stored.
-            if (u_int) {
-                const double rp = a.p_lat * ap, rn = a.n_lat * an;
-                const int pi[4] = {0, n - 1, i, i}, pj[4] = {j, j, 0, n - 1};
-                const bool has[4] = {i == 1, i == n - 2, j == 1, j == n - 2};
-                double fp = op, fn = on;
+        auto scol = [&](int ii, int jj) -> int64_t {
+            return REFLECT ? (int64_t)clampi(ii, 1, n - 2) * plane + (int64_t)clampi(jj, 1, n - 2) * rowp : acol(ii, jj);
+        };
+        auto xycol = [&](int ii, int jj) -> int64_t { return (int64_t)clampi(ii, 0, n - 1) * n + clampi(jj, 0, n - 1); };
+        const int km = max(k - 1, 0), kp = min(k + 1, nzb - 1);
+        const bool gap_k = in_gap(k), gap_km = in_gap(k - 1), gap_kp = in_gap(k + 1);
+
+        // the own column at one plane: mu_{t+1}, the stored S_t and the density
+        struct Col {
+            double ap, an, sp, sn, d;
+        };
+        auto load_col = [&](int ii) {
+            Col c;
+            const int64_t ao = acol(ii, j) + k, so = scol(ii, j) + k;
+            c.ap = a.mp[ao];
+            c.an = a.mn[ao];
+            c.sp = a.sp[so];
+            c.sn = a.sn[so];
+            c.d = a.dens[xycol(ii, j)];
+            return c;
+        };
+        // the j and k neighbours at one plane, the densities of the j
+        // neighbour columns, the column's deposition derivatives and (reflecting
+        // wall) those of its four possible ring partners
+        struct Nb {
+            double ap_jm, ap_jp, ap_km, ap_kp, an_jm, an_jp, an_km, an_kp;
+            double sp_jm, sp_jp, sp_km, sp_kp, sn_jm, sn_jp, sn_km, sn_kp;
+            double d_jm, d_jp, dgf, dh;
+            double rgf[REFLECT ? 4 : 1], rh[REFLECT ? 4 : 1];
+        };
+        auto load_nb = [&](int ii) {
+            Nb q;
+            const int64_t a_jm = acol(ii, j - 1) + k, a_jp = acol(ii, j + 1) + k, a0 = acol(ii, j);
+            const int64_t s_jm = scol(ii, j - 1) + k, s_jp = scol(ii, j + 1) + k, s0 = scol(ii, j);
+            q.ap_jm = a.mp[a_jm];
+            q.ap_jp = a.mp[a_jp];
+            q.ap_km = a.mp[a0 + km];
+            q.ap_kp = a.mp[a0 + kp];
+            q.an_jm = a.mn[a_jm];
+            q.an_jp = a.mn[a_jp];
+            q.an_km = a.mn[a0 + km];
+            q.an_kp = a.mn[a0 + kp];
+            q.sp_jm = a.sp[s_jm];
+            q.sp_jp = a.sp[s_jp];
+            q.sp_km = a.sp[s0 + km];
+            q.sp_kp = a.sp[s0 + kp];
+            q.sn_jm = a.sn[s_jm];
+            q.sn_jp = a.sn[s_jp];
+            q.sn_km = a.sn[s0 + km];
+            q.sn_kp = a.sn[s0 + kp];
+            q.d_jm = a.dens[xycol(ii, j - 1)];
+            q.d_jp = a.dens[xycol(ii, j + 1)];
+            const int64_t col = xycol(ii, j);
+            q.dgf = a.dgf[col];
+            q.dh = a.dh[col];
+            if constexpr (REFLECT) {
+                const int64_t rc[4] = {xycol(0, j), xycol(n - 1, j), xycol(ii, 0), xycol(ii, n - 1)};
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int64_t rcol = (int64_t)pi[q] * n + pj[q];
-                    const double rgf = a.dgf[rcol], rh = a.dh[rcol];
-                    fp += has[q] ? rp : 0.0;
-                    fn += has[q] ? rn : 0.0;
-                    g[9] += (has[q] && in_gap(k)) ? (rp + rn) * rgf : 0.0;
-                    g[10] += (has[q] && in_gap(k)) ? (rp + rn) * rh : 0.0;
+                for (int r = 0; r < 4; ++r) {
+                    q.rgf[r] = a.dgf[rc[r]];
+                    q.rh[r] = a.dh[rc[r]];
                 }
-                a.op[at(i, j, k)] = fp;
-                a.on[at(i, j, k)] = fn;
             }
-        } else {
-            a.op[at(i, j, k)] = op;
-            a.on[at(i, j, k)] = on;
+            return q;
+        };
+
+        // mu_t at voxel (i, j, k); adds the voxel's sensitivities to g
+        auto voxel = [&](int i, const Col &cm, const Col &c0, const Col &cpl, const Nb &q) {
+            const bool u_int = interior(i, j, k);
+            const double ap = c0.ap, an = c0.an;
+            // X_t = S_t + d_t on the gap layers
+            const double xp = c0.sp + (gap_k ? c0.d : 0.0), xn = c0.sn + (gap_k ? c0.d : 0.0);
+            const double xp_im = cm.sp + (gap_k ? cm.d : 0.0), xn_im = cm.sn + (gap_k ? cm.d : 0.0);
+            const double xp_ip = cpl.sp + (gap_k ? cpl.d : 0.0), xn_ip = cpl.sn + (gap_k ? cpl.d : 0.0);
+            const double xp_jm = q.sp_jm + (gap_k ? q.d_jm : 0.0), xn_jm = q.sn_jm + (gap_k ? q.d_jm : 0.0);
+            const double xp_jp = q.sp_jp + (gap_k ? q.d_jp : 0.0), xn_jp = q.sn_jp + (gap_k ? q.d_jp : 0.0);
+            const double xp_km = q.sp_km + (gap_km ? c0.d : 0.0), xn_km = q.sn_km + (gap_km ? c0.d : 0.0);
+            const double xp_kp = q.sp_kp + (gap_kp ? c0.d : 0.0), xn_kp = q.sn_kp + (gap_kp ? c0.d : 0.0);
+            double op = 0.0, on = 0.0;
+            if (!REFLECT) {  // carried ring and k end planes
+                op = u_int ? 0.0 : ap;
+                on = u_int ? 0.0 : an;
+            }
+            // transposed stencil: v = u + offset must be interior
+            const bool zp1 = interior(i, j, k + 1), zm1 = interior(i, j, k - 1);
+            const bool jp1 = interior(i, j + 1, k), jm1 = interior(i, j - 1, k);
+            const bool ip1 = interior(i + 1, j, k), im1 = interior(i - 1, j, k);
+            op += zp1 ? a.p_zm * q.ap_kp : 0.0;
+            op += zm1 ? a.p_zp * q.ap_km : 0.0;
+            op += jp1 ? a.p_lat * q.ap_jp : 0.0;
+            op += jm1 ? a.p_lat * q.ap_jm : 0.0;
+            op += ip1 ? a.p_lat * cpl.ap : 0.0;
+            op += im1 ? a.p_lat * cm.ap : 0.0;
+            on += zm1 ? a.n_zm * q.an_km : 0.0;
+            on += zp1 ? a.n_zp * q.an_kp : 0.0;
+            on += jp1 ? a.n_lat * q.an_jp : 0.0;
+            on += jm1 ? a.n_lat * q.an_jm : 0.0;
+            on += ip1 ? a.n_lat * cpl.an : 0.0;
+            on += im1 ? a.n_lat * cm.an : 0.0;
+            double v[kAdjNV];
+#pragma unroll
+            for (int c = 0; c < kAdjNV; ++c) v[c] = 0.0;
+            if (u_int) {
+                const double s = ap + an;
+                op += a.p_cen * ap;
+                on += a.n_cen * an;
+                op -= a.adt * xn * s;
+                on -= a.adt * xp * s;
+                const double di = (double)i - a.mid, dj = (double)j - a.mid;
+                const bool scored = gap_k && di * di + dj * dj < a.rsq;
+                op += scored ? a.adt * xn : 0.0;
+                on += scored ? a.adt * xp : 0.0;
+                // sensitivities of H (and r) at this interior voxel
+                v[0] = ap * (((xp_jm + xp_jp) + xp_im) + xp_ip);
+                v[1] = ap * xp_km;
+                v[2] = ap * xp_kp;
+                v[3] = ap * xp;
+                v[4] = an * (((xn_jm + xn_jp) + xn_im) + xn_ip);
+                v[5] = an * xn_kp;
+                v[6] = an * xn_km;
+                v[7] = an * xn;
+                const double pn = xp * xn;
+                v[8] = (scored ? pn : 0.0) - s * pn;
+            }
+            if (gap_k && (!REFLECT || u_int)) {  // (a reflecting ring's terms are taken by its partner)
+                v[9] = (op + on) * q.dgf;
+                v[10] = (op + on) * q.dh;
+            }
+            const int64_t out = acol(i, j) + k;
+            if constexpr (REFLECT) {
+                // A ring voxel next to this one reads this column (the ring is a
+                // copy of its neighbour), and its only interior consumer is this
+                // voxel, through the lateral weight: its adjoint is lat * mu(u).
+                // Its deposition term is taken here, with the ring column's
+                // dd/dgf and dd/dh2b2, and its adjoint is folded into this voxel's
+                // -- the transposed copy -- so the ring itself is never stored.
+                if (u_int) {
+                    const double rp = a.p_lat * ap, rn = a.n_lat * an;
+                    const bool has[4] = {i == 1, i == n - 2, j == 1, j == n - 2};
+                    double fp = op, fn = on;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        fp += has[r] ? rp : 0.0;
+                        fn += has[r] ? rn : 0.0;
+                        v[9] += (has[r] && gap_k) ? (rp + rn) * q.rgf[r] : 0.0;
+                        v[10] += (has[r] && gap_k) ? (rp + rn) * q.rh[r] : 0.0;
+                    }
+                    a.op[out] = fp;
+                    a.on[out] = fn;
+                }
+            } else {
+                a.op[out] = op;
+                a.on[out] = on;
+            }
+#pragma unroll
+            for (int c = 0; c < kAdjNV; ++c) g[c] += v[c];
+        };
+
+        Col cm = load_col(i0 - 1), c0 = load_col(i0), cpl = load_col(i0 + 1);
+        Nb q = load_nb(i0);
+        for (int i = i0; i < i1; ++i) {
+            const Col cn = load_col(i + 2);  // planes past the grid repeat its last one (never used)
+            const Nb qn = load_nb(i + 1);
+            voxel(i, cm, c0, cpl, q);
+            cm = c0;
+            c0 = cpl;
+            cpl = cn;
+            q = qn;
         }
     }
     // block sums of the eleven sensitivities, fixed order

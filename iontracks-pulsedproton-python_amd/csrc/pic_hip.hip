@@ -168,6 +168,7 @@ struct pic_batch {
 struct pic_adjoint {
     pic_sim *s = nullptr;
     int64_t K = 0, nseg = 0, T = 0;
+    int chunk = 2;            // i planes per adjoint block
     double *ckpt = nullptr;   // [nseg][2][n_vox]
     double *store = nullptr;  // [K + 1][2][n_vox]
     double *mu = nullptr;     // [2 buffers][2 species][n_vox]
@@ -744,8 +745,22 @@ int ensure_batch_slots(pic_batch *b, int64_t steps) {
 
 // --- adjoint helpers -------------------------------------------------
 
-int64_t adjoint_blocks(const pic_sim *s) {
-    return ceil_div(s->p.nzb, pic::kWave) * ceil_div(s->p.no_xy, 4) * s->p.no_xy;
+// i planes per adjoint block: as few as keep the grid within one wave of
+// resident blocks (each block marches its chunk; a second wave of blocks would
+// serialise behind the first), at least 2 and at most 16
+int adjoint_chunk(const pic_sim *s) {
+    int occ = 0;
+    const void *f = s->p.wall == PIC_WALL_REFLECTING ? (const void *)pic::adjoint_kernel<true>
+                                                      : (const void *)pic::adjoint_kernel<false>;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, f, 256, 0) != hipSuccess || occ < 1) occ = 1;
+    const int64_t per_plane = ceil_div(s->p.nzb, pic::kWave) * ceil_div(s->p.no_xy, 4);
+    const int64_t chunk = ceil_div(per_plane * s->p.no_xy, (int64_t)occ * s->cus);
+    return (int)std::min<int64_t>(std::max<int64_t>(chunk, 2), 16);
+}
+
+int64_t adjoint_blocks(const pic_adjoint *ad) {
+    const pic_sim *s = ad->s;
+    return ceil_div(s->p.nzb, pic::kWave) * ceil_div(s->p.no_xy, 4) * ceil_div(s->p.no_xy, ad->chunk);
 }
 
 // one forward sweep of step t from store slot src to slot src+1, with the
@@ -808,7 +823,7 @@ int adjoint_step(pic_adjoint *ad, int64_t t, int64_t src, int cur) {
     pic_sim *s = ad->s;
     const int64_t xy = s->p.no_xy * s->p.no_xy;
     const bool dep = s->offsets[t + 1] > s->offsets[t];
-    const int64_t nblk = adjoint_blocks(s);
+    const int64_t nblk = adjoint_blocks(ad);
     pic::AdjointArgs a{};
     const double *in = ad->store + src * 2 * s->n_vox;
     a.sp = in;
@@ -828,6 +843,7 @@ int adjoint_step(pic_adjoint *ad, int64_t t, int64_t src, int cur) {
     a.pitch = (int)s->pitch;
     a.n_kb = (int)ceil_div(s->p.nzb, pic::kWave);
     a.n_jb = (int)ceil_div(s->p.no_xy, 4);
+    a.chunk = ad->chunk;
     a.mid = s->p.mid_xy;
     a.rsq = s->p.scoring_radius_sq;
     a.p_lat = s->p.p_lat;
@@ -1509,7 +1525,8 @@ int pic_adjoint_create(pic_sim *s, int64_t checkpoint_every, pic_adjoint **out) 
     ad->K = std::min<int64_t>(checkpoint_every, s->batch);
     ad->T = s->n_steps;
     ad->nseg = ceil_div(ad->T, ad->K);
-    const int64_t xy = s->p.no_xy * s->p.no_xy, tiles = gather_blocks(s), nblk = adjoint_blocks(s);
+    ad->chunk = adjoint_chunk(s);
+    const int64_t xy = s->p.no_xy * s->p.no_xy, tiles = gather_blocks(s), nblk = adjoint_blocks(ad);
     const size_t pair = 2 * (size_t)s->n_vox * 8;
     const size_t sizes[] = {ad->nseg * pair, (size_t)(ad->K + 1) * pair, 2 * pair, (size_t)ad->K * xy * 8,
                             (size_t)ad->K * xy * 8, (size_t)ad->K * tiles * 2 * 8,
@@ -1584,7 +1601,7 @@ int pic_adjoint_run(pic_adjoint *ad, double sums[2], double d_rec[11], double d_
     HIP_LAUNCH_CHECK();
     hipLaunchKernelGGL(pic::fill_kernel<double>, dim3(256), dim3(256), 0, s->stream, ad->dinj_rec, 2 * T);
     HIP_LAUNCH_CHECK();
-    const int64_t nblk = adjoint_blocks(s);
+    const int64_t nblk = adjoint_blocks(ad);
     hipLaunchKernelGGL(pic::fill_kernel<double>, dim3(256), dim3(256), 0, s->stream, ad->acc,
                        nblk * 2 * pic::kAdjNV);
     HIP_LAUNCH_CHECK();
