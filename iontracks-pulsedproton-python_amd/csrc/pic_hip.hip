@@ -237,14 +237,20 @@ SweepShape compute_sweep_shape(const pic_sim *s) {
     // Enough waves for ~16 per CU, with the tallest tile (fewest halo rows) and
     // the longest chunk (fewest prologue planes) that still get there. Measured
     // on MI355X (in-pulse steps): FP32 fields best at 4-row tiles on the
-    // 1066^2 x 410 grid and 2-row tiles on 208^2 x 406; FP64 fields at 2-row
-    // tiles on both (4 rows costs occupancy); 16 planes per block throughout.
+    // 1066^2 x 410 grid and 2-row tiles on 208^2 x 406, 16 planes per block.
+    // FP64 fields: 8 planes per block; 1-row tiles while a plane of both
+    // species fits in 2 MB (their halo rows come back from the L2: 175 us on
+    // 208^2 x 406 against 207 for 2-row tiles of 16 planes), 2-row tiles above
+    // (3.99 ms on 1066^2 x 410 against 4.40 for 1-row tiles; 4 rows cost
+    // occupancy).
     const int64_t waves_wanted = (int64_t)cus * 16;
     int tj = s->force_tj;
     int chunk = s->force_chunk;
-    const int max_tj = s->prec == PIC_FP64 ? 2 : (sh.v == 1 ? 8 : 4);
+    const bool fp64_l2_plane = s->prec == PIC_FP64 && no_xy * nzb * 16 <= (int64_t)2 << 20;
+    const int max_tj = s->prec == PIC_FP64 ? (fp64_l2_plane ? 1 : 2) : (sh.v == 1 ? 8 : 4);
+    const int chunk0 = s->prec == PIC_FP64 ? 8 : 16;
     if (tj <= 0) {
-        const int c0 = chunk > 0 ? chunk : 16;
+        const int c0 = chunk > 0 ? chunk : chunk0;
         tj = 1;
         for (int c = max_tj; c >= 1; c /= 2) {
             if (ceil_div(rows, c) * sh.n_windows * ceil_div(planes, c0) >= waves_wanted) {
@@ -254,7 +260,7 @@ SweepShape compute_sweep_shape(const pic_sim *s) {
         }
     }
     if (chunk <= 0) {
-        chunk = 16;
+        chunk = chunk0;
         while (chunk > 4 && ceil_div(rows, tj) * sh.n_windows * ceil_div(planes, chunk) < waves_wanted) chunk /= 2;
     }
     sh.tj = tj;
