@@ -96,6 +96,17 @@ int pic_run_steps(pic_sim *sim, int64_t first, int64_t count);
  * first+count) to host (n x 4 doubles). Synchronises the stream. */
 int pic_read_per_step(pic_sim *sim, int64_t first, int64_t count, double *out);
 
+/* A whole run (or one pulse and its clearance) in one call: upload the track
+ * centres of n_steps steps (step s has tracks [step_offsets[s],
+ * step_offsets[s+1]) of xs/ys), run steps [0, n_steps) from the handle's
+ * current fields, and copy the (n_steps, 4) per-step record into
+ * per_step_out. Synchronous. The run-level form of the time-step loop of
+ * solver_cuda.py:791-865 with its per-step track uploads
+ * (solver_cuda.py:794-797) done once; equal to pic_upload_tracks +
+ * pic_run_steps + pic_read_per_step. */
+int pic_run_pulse(pic_sim *sim, const double *xs, const double *ys, const int64_t *step_offsets, int64_t n_steps,
+                  double *per_step_out);
+
 /* Kernel-level calls (the granularity tests/test_charge_conservation.py
  * drives _insert_track_numba / _lax_wendroff_step_numba at):
  * deposit m tracks into the current arrays and return the scored injected

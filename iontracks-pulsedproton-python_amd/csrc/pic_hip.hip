@@ -236,8 +236,11 @@ SweepShape compute_sweep_shape(const pic_sim *s) {
     const int cus = s->cus;
     // Enough waves for ~16 per CU, with the tallest tile (fewest halo rows) and
     // the longest chunk (fewest prologue planes) that still get there. Measured
-    // on MI355X (in-pulse steps): FP32 fields best at 4-row tiles on the
-    // 1066^2 x 410 grid and 2-row tiles on 208^2 x 406, 16 planes per block.
+    // on MI355X over whole runs: FP32 fields best at 4-row tiles on the
+    // 1066^2 x 410 grid and 2-row tiles on 208^2 x 406, 16 planes per block
+    // (4-row tiles there win on steps without deposition, 81.7 against 84.6 us,
+    // but lose on the depositing ones, whose 4-row kernel holds 171 VGPRs: the
+    // run's mean sweep went from 83.7 to 90.0 us).
     // FP64 fields: 8 planes per block; 1-row tiles while a plane of both
     // species fits in 2 MB (their halo rows come back from the L2: 175 us on
     // 208^2 x 406 against 207 for 2-row tiles of 16 planes), 2-row tiles above
@@ -1051,6 +1054,16 @@ int pic_read_per_step(pic_sim *s, int64_t first, int64_t count, double *out) {
     HIP_TRY(hipMemcpyAsync(out, s->per_step + 4 * first, count * 4 * 8, hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     return PIC_OK;
+}
+
+int pic_run_pulse(pic_sim *s, const double *xs, const double *ys, const int64_t *step_offsets, int64_t n_steps,
+                  double *per_step_out) {
+    if (!s || !step_offsets || !per_step_out) return fail(PIC_E_INVALID, "NULL argument");
+    if (n_steps < 1) return fail(PIC_E_INVALID, "n_steps must be positive");
+    int rc = pic_upload_tracks(s, xs, ys, step_offsets[n_steps], step_offsets, n_steps);
+    if (!rc) rc = pic_run_steps(s, 0, n_steps);
+    if (!rc) rc = pic_read_per_step(s, 0, n_steps, per_step_out);
+    return rc;
 }
 
 int pic_deposit(pic_sim *s, const double *xs, const double *ys, int64_t m, double *inserted) {

@@ -101,6 +101,20 @@ class HipSimulation:
                                             len(schedule)), "pic_upload_tracks")
         self.n_steps = len(schedule)
 
+    def run_pulse(self, schedule: np.ndarray, xs: np.ndarray, ys: np.ndarray) -> np.ndarray:
+        """Upload, run every step and return the (n_steps, 4) record in one call (pic_run_pulse)."""
+        offsets = np.zeros(len(schedule) + 1, dtype=np.int64)
+        np.cumsum(schedule, out=offsets[1:])
+        xs = np.ascontiguousarray(xs, dtype=np.float64)
+        ys = np.ascontiguousarray(ys, dtype=np.float64)
+        if len(xs) != offsets[-1] or len(ys) != offsets[-1]:
+            raise ValueError("xs/ys length does not match the schedule total")
+        out = np.zeros((len(schedule), 4))
+        _lib.check(self.L.pic_run_pulse(self.handle, _lib.dp(xs), _lib.dp(ys), _lib.ip(offsets), len(schedule),
+                                        _lib.dp(out)), "pic_run_pulse")
+        self.n_steps = len(schedule)
+        return out
+
     def run_steps(self, first: int, count: int) -> None:
         _lib.check(self.L.pic_run_steps(self.handle, int(first), int(count)), "pic_run_steps")
 

@@ -119,3 +119,26 @@ def test_fp32_fields_keep_ks_within_1e6():
     gpu = run_simulation_hip(cfg, progress=False, precision="fp32")
     assert abs(gpu.ks - ref.ks) <= 1e-6
     np.testing.assert_array_equal(gpu.track_density_xy, ref.track_density_xy)
+
+
+def test_run_pulse_equals_upload_run_read():
+    """pic_run_pulse (one call) gives the record of pic_upload_tracks +
+    pic_run_steps + pic_read_per_step bit for bit, and leaves the same fields."""
+    from configs import readme_config
+    from pulsed_ion_chamber_hip.pulses import draw_tracks
+    from pulsed_ion_chamber_hip.solver_hip import HipSimulation
+
+    cfg = readme_config(20.0, 0.0101, voltage_V=96.53, dose_rate_Gy_s=40.0)  # 18^2 x 106, 337 steps
+    draw = draw_tracks(cfg)
+    with HipSimulation(cfg, "fp64") as sim:
+        one = sim.run_pulse(*draw)
+        f_one = sim.read_fields()
+    with HipSimulation(cfg, "fp64") as sim:
+        sim.upload_tracks(*draw)
+        sim.run_steps(0, cfg.total_time_steps)
+        three = sim.read_per_step()
+        f_three = sim.read_fields()
+    np.testing.assert_array_equal(one, three)
+    for a, b in zip(f_one, f_three):
+        np.testing.assert_array_equal(a, b)
+    assert one[:, 0].sum() > 0
