@@ -239,18 +239,16 @@ SweepShape compute_sweep_shape(const pic_sim *s) {
     // on MI355X over whole runs: FP32 fields best at 4-row tiles on the
     // 1066^2 x 410 grid and 2-row tiles on 208^2 x 406, 16 planes per block
     // (4-row tiles there win on steps without deposition, 81.7 against 84.6 us,
-    // but lose on the depositing ones, whose 4-row kernel holds 171 VGPRs: the
-    // run's mean sweep went from 83.7 to 90.0 us).
-    // FP64 fields: 8 planes per block; 1-row tiles while a plane of both
-    // species fits in 2 MB (their halo rows come back from the L2: 175 us on
-    // 208^2 x 406 against 207 for 2-row tiles of 16 planes), 2-row tiles above
-    // (3.99 ms on 1066^2 x 410 against 4.40 for 1-row tiles; 4 rows cost
-    // occupancy).
+    // but lose on the depositing ones, whose 4-row kernel needs 169 VGPRs and
+    // runs two waves per SIMD: the run's mean sweep is 84.2 us with 2-row and
+    // 89.0 with 4-row tiles).
+    // FP64 fields: 2-row tiles of 8 planes (163 us on 208^2 x 406 against 169
+    // for 16 planes and 185 for 1-row tiles; 3.99 ms on 1066^2 x 410 against
+    // 4.12 for 16 planes and 4.40 for 1-row tiles; 4 rows cost occupancy).
     const int64_t waves_wanted = (int64_t)cus * 16;
     int tj = s->force_tj;
     int chunk = s->force_chunk;
-    const bool fp64_l2_plane = s->prec == PIC_FP64 && no_xy * nzb * 16 <= (int64_t)2 << 20;
-    const int max_tj = s->prec == PIC_FP64 ? (fp64_l2_plane ? 1 : 2) : (sh.v == 1 ? 8 : 4);
+    const int max_tj = s->prec == PIC_FP64 ? 2 : (sh.v == 1 ? 8 : 4);
     const int chunk0 = s->prec == PIC_FP64 ? 8 : 16;
     if (tj <= 0) {
         const int c0 = chunk > 0 ? chunk : chunk0;

@@ -376,14 +376,19 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &a, const int logical
                     (NO + row)[k0 + v] = vn[v];
                 }
         };
-        // absorbing wall: carry rows r of a loaded plane that lie on the ring
+        // absorbing wall: carry the rows of a loaded plane that lie on the ring
+        // (every own row too when the whole plane is ring). Every buffer index
+        // is a constant: the row after the last own one, j0 + nrows, is always
+        // in slot TJ + 1 (load_rows repeats it past the tile), so no register
+        // array is indexed at run time -- which would move it to scratch memory.
         auto carry_ring_rows = [&](int i, const S (&rp)[TJ + 2][V], const S (&rn)[TJ + 2][V], bool whole_plane) {
+            if (whole_plane) {
 #pragma unroll
-            for (int r = 0; r < TJ + 2; ++r) {
-                const bool own = r >= 1 && r <= nrows;
-                const bool ring_row = (r == 0 && first_tile) || (r == nrows + 1 && last_tile);
-                if ((whole_plane && own) || ring_row) store2(i, j0 + r - 1, rp[r], rn[r]);
+                for (int r = 1; r <= TJ; ++r)
+                    if (r <= nrows) store2(i, j0 + r - 1, rp[r], rn[r]);
             }
+            if (first_tile) store2(i, j0 - 1, rp[0], rn[0]);
+            if (last_tile) store2(i, j0 + nrows, rp[TJ + 1], rn[TJ + 1]);
         };
 
         // Plane buffers in rotation, NB = D + 3 of them for a lookahead of D
