@@ -18,6 +18,8 @@
 #include <cstdio>
 #include <cstring>
 #include <mutex>
+#include <string>
+#include <thread>
 #include <vector>
 
 #include "pic_hip.h"
@@ -159,6 +161,15 @@ struct pic_batch {
     size_t ev_used = 0;
     bool graphs = false;       // replay each segment's launches through graph chains (never while timing)
     std::vector<KLaunch> seg;  // one segment's launches
+    // Long-run lane: the configurations that run much longer than the others
+    // (over 0.8 of the longest run, at most a quarter of them) are advanced by
+    // a sub-batch of their own, on its own stream and host thread, beside a
+    // sub-batch of the rest; their latency-bound tail of small launches then
+    // overlaps the rest's sweeps instead of following them. The sub-batches
+    // are built on the first run that splits and kept while the split holds.
+    pic_batch *lanes[2] = {nullptr, nullptr};
+    std::vector<int> lane_of;  // the split the lanes were built for: lane of sims[c]
+    bool is_lane = false;      // a sub-batch: never splits again
 };
 
 // Reverse-mode derivative of one handle's run: checkpoints of the state
@@ -750,6 +761,82 @@ int ensure_batch_slots(pic_batch *b, int64_t steps) {
     return PIC_OK;
 }
 
+// --- batch lanes -----------------------------------------------------
+
+// The long-run lane of a batch: part[c] = 0 for the configurations running
+// over 0.8 of the longest run, 1 for the others. A split needs at least four
+// configurations and puts at most a quarter of them in the long lane; false
+// when there is no such split. (Measured on the 32-point saturation curve:
+// the longest one to three runs in a lane of their own take the batch from
+// 38.9 to 34.7-35.3 ms; four to twelve, 36.3-37.1 ms; three lanes, 44 ms.)
+bool lane_split(const pic_batch *b, std::vector<int> &part) {
+    const int n = (int)b->sims.size();
+    if (n < 4) return false;
+    int64_t longest = 0;
+    for (const pic_sim *s : b->sims) longest = std::max(longest, s->n_steps);
+    part.assign(n, 1);
+    int n_long = 0;
+    for (int c = 0; c < n; ++c)
+        if ((double)b->sims[c]->n_steps > 0.8 * (double)longest) {
+            part[c] = 0;
+            ++n_long;
+        }
+    return n_long >= 1 && n_long <= n / 4;
+}
+
+// Run steps [first, first+count) as two lanes (sub-batches over the split
+// `part`), each driven by its own host thread on its own stream; the batch's
+// stream then waits for both, and each handle's stream for its lane.
+int run_lanes(pic_batch *b, const std::vector<int> &part, int64_t first, int64_t count) {
+    if (part != b->lane_of || !b->lanes[0] || !b->lanes[1]) {
+        for (pic_batch *&l : b->lanes) {
+            pic_batch_destroy(l);
+            l = nullptr;
+        }
+        b->lane_of.clear();
+        for (int L = 0; L < 2; ++L) {
+            std::vector<pic_sim *> members;
+            for (size_t c = 0; c < b->sims.size(); ++c)
+                if (part[c] == L) members.push_back(b->sims[c]);
+            int rc = pic_batch_create(members.data(), (int)members.size(), &b->lanes[L]);
+            if (!rc && (b->force_tj > 0 || b->force_chunk > 0))
+                rc = pic_batch_set_tiling(b->lanes[L], b->force_tj, b->force_chunk);
+            if (rc) return rc;
+            b->lanes[L]->is_lane = true;
+            b->lanes[L]->graphs = b->graphs;
+        }
+        b->lane_of = part;
+    }
+    int rc[2] = {PIC_OK, PIC_OK};
+    std::string err[2];
+    auto go = [&](int L) {
+        pic_batch *l = b->lanes[L];
+        int64_t longest = 0;
+        for (const pic_sim *s : l->sims) longest = std::max(longest, s->n_steps);
+        const int64_t c = std::min(first + count, longest) - first;
+        if (c > 0) rc[L] = pic_batch_run(l, first, c);
+        if (rc[L]) err[L] = g_err;
+    };
+    const int device = b->sims[0]->device;
+    std::thread worker([&] {
+        if (hipSetDevice(device) != hipSuccess) {
+            rc[0] = fail(PIC_E_HIP, "hipSetDevice failed in the lane thread");
+            err[0] = g_err;
+            return;
+        }
+        go(0);
+    });
+    go(1);
+    worker.join();
+    for (int L = 0; L < 2; ++L)
+        if (rc[L]) {
+            snprintf(g_err, sizeof g_err, "%s", err[L].c_str());
+            return rc[L];
+        }
+    for (pic_batch *l : b->lanes) HIP_TRY(hipStreamWaitEvent(b->stream, l->done, 0));
+    return PIC_OK;
+}
+
 // --- adjoint helpers -------------------------------------------------
 
 // i planes per adjoint block: as few as keep the grid within one wave of
@@ -1312,6 +1399,10 @@ int pic_batch_create(pic_sim *const *sims, int n, pic_batch **out) {
 
 int pic_batch_destroy(pic_batch *b) {
     if (!b) return PIC_OK;
+    for (pic_batch *&l : b->lanes) {
+        pic_batch_destroy(l);
+        l = nullptr;
+    }
     if (b->stream) hipStreamSynchronize(b->stream);
     free_batch_slots(b);
     if (b->zero_dens) hipFree(b->zero_dens);
@@ -1331,6 +1422,11 @@ int pic_batch_set_tiling(pic_batch *b, int rows_per_tile, int planes_per_chunk) 
     if (rows_per_tile == 4 && b->prec == PIC_FP64) return fail(PIC_E_INVALID, "batched FP64 sweeps take 1 or 2 rows per tile");
     if (planes_per_chunk < 0) return fail(PIC_E_INVALID, "negative planes_per_chunk");
     HIP_TRY(hipStreamSynchronize(b->stream));
+    for (pic_batch *&l : b->lanes) {  // rebuilt with the new tiling at the next split run
+        pic_batch_destroy(l);
+        l = nullptr;
+    }
+    b->lane_of.clear();
     const int old_tj = b->force_tj, old_chunk = b->force_chunk;
     b->force_tj = rows_per_tile;
     b->force_chunk = planes_per_chunk;
@@ -1361,6 +1457,10 @@ int pic_batch_run(pic_batch *b, int64_t first, int64_t count) {
     if (first < 0 || count < 0 || first + count > longest)
         return fail(PIC_E_INVALID, "steps [%lld, %lld) outside the longest uploaded run (%lld)", (long long)first,
                     (long long)(first + count), (long long)longest);
+    if (!b->is_lane && !b->timing) {
+        std::vector<int> part;
+        if (lane_split(b, part)) return run_lanes(b, part, first, count);
+    }
     // run order: longest first, so the configurations still running at any
     // step are a prefix and the grid covers exactly their blocks
     std::vector<int> ord(n);
@@ -1499,6 +1599,8 @@ int pic_batch_enable_timing(pic_batch *b, int enable) {
 int pic_batch_set_graphs(pic_batch *b, int enable) {
     if (!b) return fail(PIC_E_INVALID, "batch is NULL");
     b->graphs = enable != 0;
+    for (pic_batch *l : b->lanes)
+        if (l) l->graphs = b->graphs;
     return PIC_OK;
 }
 
