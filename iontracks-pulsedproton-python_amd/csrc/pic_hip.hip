@@ -818,16 +818,23 @@ int run_lanes(pic_batch *b, const std::vector<int> &part, int64_t first, int64_t
         if (rc[L]) err[L] = g_err;
     };
     const int device = b->sims[0]->device;
-    std::thread worker([&] {
-        if (hipSetDevice(device) != hipSuccess) {
-            rc[0] = fail(PIC_E_HIP, "hipSetDevice failed in the lane thread");
-            err[0] = g_err;
-            return;
-        }
-        go(0);
-    });
+    std::thread worker;
+    bool threaded = true;
+    try {
+        worker = std::thread([&] {
+            if (hipSetDevice(device) != hipSuccess) {
+                rc[0] = fail(PIC_E_HIP, "hipSetDevice failed in the lane thread");
+                err[0] = g_err;
+                return;
+            }
+            go(0);
+        });
+    } catch (...) {  // no thread to be had: both lanes from this one, one after the other
+        threaded = false;
+    }
     go(1);
-    worker.join();
+    if (threaded) worker.join();
+    else go(0);
     for (int L = 0; L < 2; ++L)
         if (rc[L]) {
             snprintf(g_err, sizeof g_err, "%s", err[L].c_str());
