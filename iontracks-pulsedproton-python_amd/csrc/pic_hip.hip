@@ -245,22 +245,19 @@ SweepShape compute_sweep_shape(const pic_sim *s) {
     sh.n_kg = (int)ceil_div(sh.n_windows, sh.wpb);
     const int64_t rows = no_xy - 2, planes = no_xy - 2;
     const int cus = s->cus;
-    // Enough waves for ~16 per CU, with the tallest tile (fewest halo rows) and
-    // the longest chunk (fewest prologue planes) that still get there. Measured
-    // on MI355X over whole runs: FP32 fields best at 4-row tiles on the
-    // 1066^2 x 410 grid and 2-row tiles on 208^2 x 406, 16 planes per block
-    // (4-row tiles there win on steps without deposition, 81.7 against 84.6 us,
-    // but lose on the depositing ones, whose 4-row kernel needs 169 VGPRs and
-    // runs two waves per SIMD: the run's mean sweep is 84.2 us with 2-row and
-    // 89.0 with 4-row tiles).
-    // FP64 fields: 2-row tiles of 8 planes (163 us on 208^2 x 406 against 169
-    // for 16 planes and 185 for 1-row tiles; 3.99 ms on 1066^2 x 410 against
-    // 4.12 for 16 planes and 4.40 for 1-row tiles; 4 rows cost occupancy).
-    const int64_t waves_wanted = (int64_t)cus * 16;
+    // Enough waves for ~8 per CU, with the tallest tile (fewest halo rows) and
+    // the longest chunk (fewest prologue planes) that still get there.
+    // Measured on MI355X over whole runs (depositing steps included), with the
+    // buffer-addressed sweep: 4-row tiles of 16 planes are best for both
+    // precisions on 208^2 x 406 (FP32 80.5 us, against 84.4 for 2-row tiles;
+    // FP64 154 us, against 162 for 2-row tiles of 8 planes) and on
+    // 1066^2 x 410 (FP32 1.88 ms; FP64 3.65 ms, against 4.02 for 2-row tiles
+    // of 8 planes). Small grids fall through to 1-row tiles of 4 planes.
+    const int64_t waves_wanted = (int64_t)cus * 8;
     int tj = s->force_tj;
     int chunk = s->force_chunk;
-    const int max_tj = s->prec == PIC_FP64 ? 2 : (sh.v == 1 ? 8 : 4);
-    const int chunk0 = s->prec == PIC_FP64 ? 8 : 16;
+    const int max_tj = (sh.v == 1 && s->prec == PIC_FP32) ? 8 : 4;
+    const int chunk0 = 16;
     if (tj <= 0) {
         const int c0 = chunk > 0 ? chunk : chunk0;
         tj = 1;
@@ -666,6 +663,11 @@ int check_params(const pic_params *p, int precision) {
     if (p->nze < 0 || p->no_z < 0 || p->nze + p->no_z > p->nzb) return fail(PIC_E_INVALID, "gap layers outside the grid");
     if (p->wall != PIC_WALL_ABSORBING && p->wall != PIC_WALL_REFLECTING) return fail(PIC_E_INVALID, "wall must be PIC_WALL_ABSORBING or PIC_WALL_REFLECTING");
     if (!(p->cutoff_voxels > 0.0)) return fail(PIC_E_INVALID, "cutoff_voxels must be positive");
+    // the sweep addresses one i plane of an array through a buffer resource
+    // with 32-bit offsets
+    const int64_t plane_bytes = p->no_xy * row_pitch(p, precision) * (precision == PIC_FP64 ? 8 : 4);
+    if (plane_bytes >= INT32_MAX)
+        return fail(PIC_E_INVALID, "one i plane (%lld x %lld) exceeds 2 GB", (long long)p->no_xy, (long long)p->nzb);
     return PIC_OK;
 }
 

@@ -190,6 +190,11 @@ struct SweepArgs {
 };
 
 typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+
+// Fourth word of a raw buffer resource on gfx9-family devices (gfx950
+// included): 32-bit data format, no swizzle, no index stride.
+constexpr int kBufferWord3 = 0x00020000;
 
 // One Lax-Wendroff step of both species, fused with the gap-layer deposition
 // of this step (DEP), the lateral-ring update of the next arrays and the
@@ -296,21 +301,39 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &a, const int logical
         // wave waits only for the plane it is about to use while the planes
         // behind it stay in flight; a conditional load would force it to drain
         // every load at each use.
+        //
+        // Every field access is a buffer access over one i plane: a resource
+        // whose base is the plane (wave-uniform, scalar registers), the row
+        // as the scalar offset and the lane's k as a 32-bit vector offset, so
+        // no access spends vector instructions on a 64-bit address (the host
+        // keeps a plane below 2 GB).
         const int kc = k_in ? k0 : pitch - V;
+        const unsigned vo_load = (unsigned)kc * (unsigned)sizeof(S), vo_store = (unsigned)k0 * (unsigned)sizeof(S);
+        const int plane_bytes = (int)(plane * (int64_t)sizeof(S));
+        auto plane_rsrc = [&](const void *arr, int i) {
+            return __builtin_amdgcn_make_buffer_rsrc(
+                const_cast<void *>(static_cast<const void *>(static_cast<const S *>(arr) + (int64_t)i * plane)), 0,
+                plane_bytes, kBufferWord3);
+        };
+        auto row_off = [&](int j) -> int { return (int)((int64_t)j * pitch * (int64_t)sizeof(S)); };
         auto load_raw = [&](int i, int j, S (&vp)[V], S (&vn)[V]) {
             const int si = REFLECT ? min(max(i, 1), no_xy - 2) : i;
             const int sj = REFLECT ? min(max(j, 1), no_xy - 2) : j;
-            const int64_t row = (int64_t)si * plane + (int64_t)sj * pitch;
+            const auto rp = plane_rsrc(P, si), rn = plane_rsrc(N, si);
+            const int so = row_off(sj);
             if constexpr (V == 2) {
-                const f32x2 tp = *reinterpret_cast<const f32x2 *>(P + row + kc);
-                const f32x2 tn = *reinterpret_cast<const f32x2 *>(N + row + kc);
+                const f32x2 tp = __builtin_bit_cast(f32x2, __builtin_amdgcn_raw_buffer_load_b64(rp, vo_load, so, 0));
+                const f32x2 tn = __builtin_bit_cast(f32x2, __builtin_amdgcn_raw_buffer_load_b64(rn, vo_load, so, 0));
                 vp[0] = tp.x;
                 vp[1] = tp.y;
                 vn[0] = tn.x;
                 vn[1] = tn.y;
+            } else if constexpr (sizeof(S) == 8) {
+                vp[0] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rp, vo_load, so, 0));
+                vn[0] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rn, vo_load, so, 0));
             } else {
-                vp[0] = (P + row)[kc];
-                vn[0] = (N + row)[kc];
+                vp[0] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rp, vo_load, so, 0));
+                vn[0] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rn, vo_load, so, 0));
             }
         };
         // This step's deposition of the TJ+2 rows of a plane: one vector load in
@@ -361,19 +384,26 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &a, const int logical
         };
         // store V values of (i, j) to the next arrays, lanes/values that own k
         auto store2 = [&](int i, int j, const S (&vp)[V], const S (&vn)[V]) {
-            const int64_t row = (int64_t)i * plane + (int64_t)j * pitch;
+            const auto rp = plane_rsrc(PO, i), rn = plane_rsrc(NO, i);
+            const int so = row_off(j);
             if constexpr (V == 2) {
                 if (k_out[0] && k_out[1]) {
-                    *reinterpret_cast<f32x2 *>(PO + row + k0) = f32x2{vp[0], vp[1]};
-                    *reinterpret_cast<f32x2 *>(NO + row + k0) = f32x2{vn[0], vn[1]};
+                    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, f32x2{vp[0], vp[1]}), rp, vo_store, so, 0);
+                    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, f32x2{vn[0], vn[1]}), rn, vo_store, so, 0);
                     return;
                 }
             }
 #pragma unroll
             for (int v = 0; v < V; ++v)
                 if (k_out[v]) {
-                    (PO + row)[k0 + v] = vp[v];
-                    (NO + row)[k0 + v] = vn[v];
+                    const unsigned vo = vo_store + (unsigned)(v * sizeof(S));
+                    if constexpr (sizeof(S) == 8) {
+                        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, vp[v]), rp, vo, so, 0);
+                        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, vn[v]), rn, vo, so, 0);
+                    } else {
+                        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, vp[v]), rp, vo, so, 0);
+                        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, vn[v]), rn, vo, so, 0);
+                    }
                 }
         };
         // absorbing wall: carry the rows of a loaded plane that lie on the ring

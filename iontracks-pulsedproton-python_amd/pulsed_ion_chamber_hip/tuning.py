@@ -50,7 +50,7 @@ def candidates(precision: str) -> list:
     out = []
     lanes = (2, 1) if precision == "fp32" else (1,)
     for v in lanes:
-        rows = (1, 2, 4, 8) if (precision == "fp32" and v == 1) else (1, 2, 4) if precision == "fp32" else (1, 2)
+        rows = (1, 2, 4, 8) if (precision == "fp32" and v == 1) else (1, 2, 4)
         for tj in rows:
             for chunk in (4, 8, 16, 32):
                 out.append((tj, chunk, v))
