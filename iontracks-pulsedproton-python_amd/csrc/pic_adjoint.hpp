@@ -115,7 +115,9 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjointArgs a) {
     const int jb = rest % a.n_jb;
     const int ic = rest / a.n_jb;
     const int k = kb * kWave + (threadIdx.x & 63);
-    const int j = jb * 4 + (threadIdx.x >> 6);
+    // the row index is uniform across a wave; saying so keeps the row parts of
+    // the addresses in scalar registers
+    const int j = jb * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int n = a.no_xy, nzb = a.nzb;
     double g[kAdjNV];
 #pragma unroll
@@ -140,6 +142,19 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjointArgs a) {
         };
         auto xycol = [&](int ii, int jj) -> int64_t { return (int64_t)clampi(ii, 0, n - 1) * n + clampi(jj, 0, n - 1); };
         const int km = max(k - 1, 0), kp = min(k + 1, nzb - 1);
+        // The four state arrays through buffer resources: the (wave-uniform)
+        // column offset is the scalar offset and the lane's k a 32-bit vector
+        // offset, so no load spends vector instructions on a 64-bit address
+        // (the host keeps an array below 2 GB).
+        const int arr_bytes = (int)((int64_t)n * n * a.pitch * 8);
+        auto rsrc = [&](const double *q) {
+            return __builtin_amdgcn_make_buffer_rsrc(const_cast<double *>(q), 0, arr_bytes, kBufferWord3);
+        };
+        const auto r_mp = rsrc(a.mp), r_mn = rsrc(a.mn), r_sp = rsrc(a.sp), r_sn = rsrc(a.sn);
+        auto ld = [](__amdgpu_buffer_rsrc_t r, int64_t col, int kk) -> double {
+            return __builtin_bit_cast(double,
+                                      __builtin_amdgcn_raw_buffer_load_b64(r, (unsigned)kk * 8u, (int)(col * 8), 0));
+        };
         const bool gap_k = in_gap(k), gap_km = in_gap(k - 1), gap_kp = in_gap(k + 1);
 
         // the own column at one plane: mu_{t+1}, the stored S_t and the density
@@ -148,11 +163,11 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjointArgs a) {
         };
         auto load_col = [&](int ii) {
             Col c;
-            const int64_t ao = acol(ii, j) + k, so = scol(ii, j) + k;
-            c.ap = a.mp[ao];
-            c.an = a.mn[ao];
-            c.sp = a.sp[so];
-            c.sn = a.sn[so];
+            const int64_t ao = acol(ii, j), so = scol(ii, j);
+            c.ap = ld(r_mp, ao, k);
+            c.an = ld(r_mn, ao, k);
+            c.sp = ld(r_sp, so, k);
+            c.sn = ld(r_sn, so, k);
             c.d = a.dens[xycol(ii, j)];
             return c;
         };
@@ -167,24 +182,24 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjointArgs a) {
         };
         auto load_nb = [&](int ii) {
             Nb q;
-            const int64_t a_jm = acol(ii, j - 1) + k, a_jp = acol(ii, j + 1) + k, a0 = acol(ii, j);
-            const int64_t s_jm = scol(ii, j - 1) + k, s_jp = scol(ii, j + 1) + k, s0 = scol(ii, j);
-            q.ap_jm = a.mp[a_jm];
-            q.ap_jp = a.mp[a_jp];
-            q.ap_km = a.mp[a0 + km];
-            q.ap_kp = a.mp[a0 + kp];
-            q.an_jm = a.mn[a_jm];
-            q.an_jp = a.mn[a_jp];
-            q.an_km = a.mn[a0 + km];
-            q.an_kp = a.mn[a0 + kp];
-            q.sp_jm = a.sp[s_jm];
-            q.sp_jp = a.sp[s_jp];
-            q.sp_km = a.sp[s0 + km];
-            q.sp_kp = a.sp[s0 + kp];
-            q.sn_jm = a.sn[s_jm];
-            q.sn_jp = a.sn[s_jp];
-            q.sn_km = a.sn[s0 + km];
-            q.sn_kp = a.sn[s0 + kp];
+            const int64_t a_jm = acol(ii, j - 1), a_jp = acol(ii, j + 1), a0 = acol(ii, j);
+            const int64_t s_jm = scol(ii, j - 1), s_jp = scol(ii, j + 1), s0 = scol(ii, j);
+            q.ap_jm = ld(r_mp, a_jm, k);
+            q.ap_jp = ld(r_mp, a_jp, k);
+            q.ap_km = ld(r_mp, a0, km);
+            q.ap_kp = ld(r_mp, a0, kp);
+            q.an_jm = ld(r_mn, a_jm, k);
+            q.an_jp = ld(r_mn, a_jp, k);
+            q.an_km = ld(r_mn, a0, km);
+            q.an_kp = ld(r_mn, a0, kp);
+            q.sp_jm = ld(r_sp, s_jm, k);
+            q.sp_jp = ld(r_sp, s_jp, k);
+            q.sp_km = ld(r_sp, s0, km);
+            q.sp_kp = ld(r_sp, s0, kp);
+            q.sn_jm = ld(r_sn, s_jm, k);
+            q.sn_jp = ld(r_sn, s_jp, k);
+            q.sn_km = ld(r_sn, s0, km);
+            q.sn_kp = ld(r_sn, s0, kp);
             q.d_jm = a.dens[xycol(ii, j - 1)];
             q.d_jp = a.dens[xycol(ii, j + 1)];
             const int64_t col = xycol(ii, j);

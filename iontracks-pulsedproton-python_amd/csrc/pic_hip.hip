@@ -1646,6 +1646,8 @@ int pic_adjoint_create(pic_sim *s, int64_t checkpoint_every, pic_adjoint **out) 
     if (s->p.no_xy < 4) return fail(PIC_E_INVALID, "the adjoint needs no_xy >= 4");
     if (s->n_steps <= 0) return fail(PIC_E_INVALID, "upload the tracks before creating the adjoint");
     if (checkpoint_every < 1) return fail(PIC_E_INVALID, "checkpoint_every must be positive");
+    if (s->n_vox * 8 >= INT32_MAX)  // the adjoint kernel addresses each array through a 32-bit buffer offset
+        return fail(PIC_E_INVALID, "the adjoint needs arrays below 2 GB (this grid has %lld voxels)", (long long)s->n_vox);
     pic_adjoint *ad = new pic_adjoint();
     ad->s = s;
     // a segment's densities must stay in the handle's gather slots until its
