@@ -212,7 +212,7 @@ int values_per_lane(const pic_sim *s, bool force_v1) { return (s->prec == PIC_FP
 SweepShape fixed_shape(const pic_sim *s, int v, int tj, int chunk, int wpb) {
     SweepShape sh;
     sh.v = v;
-    sh.n_windows = (int)ceil_div(s->p.nzb - 2, pic::kWave * v - 2);
+    sh.n_windows = (int)ceil_div(s->p.nzb - 2, pic::window_outputs(v));
     sh.wpb = wpb;
     sh.n_kg = (int)ceil_div(sh.n_windows, wpb);
     const int64_t rows = s->p.no_xy - 2, planes = s->p.no_xy - 2;
@@ -240,7 +240,7 @@ SweepShape compute_sweep_shape(const pic_sim *s) {
     SweepShape sh;
     const int64_t no_xy = s->p.no_xy, nzb = s->p.nzb;
     sh.v = values_per_lane(s, s->force_v1);
-    sh.n_windows = (int)ceil_div(nzb - 2, pic::kWave * sh.v - 2);
+    sh.n_windows = (int)ceil_div(nzb - 2, pic::window_outputs(sh.v));
     sh.wpb = std::min(sh.n_windows, 8);
     sh.n_kg = (int)ceil_div(sh.n_windows, sh.wpb);
     const int64_t rows = no_xy - 2, planes = no_xy - 2;
@@ -697,7 +697,7 @@ int batch_tiling(pic_batch *b) {
     int v = 2, wpb = 8;
     for (pic_sim *s : b->sims) v = std::min(v, values_per_lane(s, false));
     for (pic_sim *s : b->sims)
-        wpb = std::min(wpb, (int)ceil_div(s->p.nzb - 2, pic::kWave * v - 2));
+        wpb = std::min(wpb, (int)ceil_div(s->p.nzb - 2, pic::window_outputs(v)));
     const int cus = b->sims[0]->cus;
     std::vector<std::pair<int, int>> cands;
     if (b->force_tj > 0 || b->force_chunk > 0)

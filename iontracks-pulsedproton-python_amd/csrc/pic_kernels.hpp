@@ -16,6 +16,11 @@ namespace pic {
 
 constexpr int kWave = 64;
 constexpr int kOut = 62;  // output lanes per wave window (lanes 1..62)
+
+// k values a wave window writes: one value per lane, lanes 1..62; two values
+// per lane, the pairs of lanes 1..62 (windows step by 124 and overlap by four,
+// so every storing lane writes a whole 8-byte pair)
+constexpr int window_outputs(int v) { return v == 2 ? 2 * kWave - 4 : kWave - 2; }
 constexpr int kXcds = 8;  // MI355X: blocks are dispatched round-robin over 8 XCDs
 
 // Map the hardware block index to a logical tile index so that each XCD
@@ -182,7 +187,7 @@ struct SweepArgs {
     double *prev_out4;            // (injected, recombined, total+, total-), or null
     int no_xy, nzb, nze, no_z;
     int pitch;      // elements from one (i, j) row to the next
-    int n_windows;  // k windows of kOut outputs covering k = 1..nzb-2
+    int n_windows;  // k windows of window_outputs(V) outputs covering k = 1..nzb-2
     int wpb;        // windows (waves) per block
     int n_kg, n_jt, n_ic;
     int chunk;  // planes per block
@@ -206,9 +211,13 @@ constexpr int kBufferWord3 = 0x00020000;
 // Work decomposition: a wave owns a k window, TJ consecutive interior rows j of
 // one tile, and marches through a chunk of i planes. Each lane holds V
 // consecutive k values (V = 2 for FP32 fields, one 8-byte load per row, so a
-// wave moves 512 B per load as the FP64 path does with V = 1); a window of
-// 64*V values writes all but its first and last one, and consecutive windows
-// overlap by two so those edge values are supplied by the neighbour window.
+// wave moves 512 B per load as the FP64 path does with V = 1). The values of
+// lanes 0 and 63 only feed their neighbours' stencils, so the window writes
+// lanes 1..62 (window_outputs) and consecutive windows overlap by 2V, the
+// first window's lane 0 also writing k = 1. With V = 2 each storing lane then
+// writes a whole pair -- one 8-byte store per row and array, no partial-lane
+// stores -- and the one value of a pair it does not own is a k end plane,
+// which stays zero.
 // Per lane it keeps, in registers, the tile's rows of plane i-1, plane i (with
 // the two halo rows j0-1 and j0+TJ), plane i+1, and plane i+2 which is already
 // being loaded while plane i is computed, so memory latency overlaps the
@@ -253,7 +262,7 @@ constexpr int kBufferWord3 = 0x00020000;
 template <typename S, typename A, int TJ, int V, bool REFLECT>
 __device__ __forceinline__ void sweep_body(const SweepArgs &a, const int logical, const bool DEP) {
     static_assert(V == 1 || (V == 2 && sizeof(S) == 4), "two values per lane is the FP32 layout");
-    constexpr int WIN = kWave * V - 2;  // outputs per window
+    constexpr int WIN = window_outputs(V);  // k step between windows
     const int kg = logical % a.n_kg;
     const int rest = logical / a.n_kg;
     const int jt = rest % a.n_jt;
@@ -272,7 +281,9 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &a, const int logical
 #pragma unroll
         for (int v = 0; v < V; ++v) {
             const int q = V * lane + v, k = k0 + v;
-            k_out[v] = q >= 1 && q <= kWave * V - 2 && k >= 1 && k <= nzb - 2;
+            // owned: lanes 1..62 (and k = 1 in the first window), inside the k range
+            const bool own_lane = V == 2 ? (q >= 2 || win == 0) && q <= 2 * kWave - 3 : q >= 1 && q <= kWave - 2;
+            k_out[v] = own_lane && k >= 1 && k <= nzb - 2;
             k_gap[v] = k >= a.nze && k < a.nze + a.no_z;
             k_score[v] = k_out[v] && k_gap[v];
         }
@@ -390,11 +401,15 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &a, const int logical
             const auto rp = plane_rsrc(PO, i), rn = plane_rsrc(NO, i);
             const int so = row_off(j);
             if constexpr (V == 2) {
-                if (k_out[0] && k_out[1]) {
-                    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, f32x2{vp[0], vp[1]}), rp, vo_store, so, 0);
-                    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, f32x2{vn[0], vn[1]}), rn, vo_store, so, 0);
-                    return;
+                // the whole pair: a value the lane does not own is k = 0 or
+                // nzb-1, a k end plane, which must stay zero
+                if (k_out[0] || k_out[1]) {
+                    const f32x2 wp = {k_out[0] ? vp[0] : 0.0f, k_out[1] ? vp[1] : 0.0f};
+                    const f32x2 wn = {k_out[0] ? vn[0] : 0.0f, k_out[1] ? vn[1] : 0.0f};
+                    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, wp), rp, vo_store, so, 0);
+                    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, wn), rn, vo_store, so, 0);
                 }
+                return;
             }
 #pragma unroll
             for (int v = 0; v < V; ++v)

@@ -103,6 +103,11 @@ def test_deposit_matches_the_batched_oracle_and_is_deterministic():
 # (the device pads FP32 rows to an even length)
 GRIDS_UM = [40.0, 16.0]
 
+# FP32 k-window layouts, as (grid um, electrode gap cm): one window, an odd
+# row length, 16 x 16 x 254 (252 interior k values: two windows of 126 but
+# three of the 124 a two-value window writes) and 22 x 22 x 406 (four windows)
+FP32_GRIDS = [(40.0, 0.2), (16.0, 0.2), (8.0, 0.1984), (5.0, 0.2)]
+
 
 @pytest.mark.parametrize("grid_um", GRIDS_UM)
 @pytest.mark.parametrize("wall", ["absorbing", "reflecting"])
@@ -147,10 +152,11 @@ def test_one_fp64_sweep_is_bit_identical_to_the_oracle(wall, species, grid_um):
             np.testing.assert_array_equal(a[:, -1], m[:, -1])
 
 
-@pytest.mark.parametrize("grid_um", GRIDS_UM)
+@pytest.mark.parametrize("grid_um,gap_cm", FP32_GRIDS)
 @pytest.mark.parametrize("wall", ["absorbing", "reflecting"])
-def test_one_fp32_sweep_stays_within_fp32_rounding_of_the_oracle(wall, grid_um):
-    config = SimulationConfig(**{**CC, "grid_size_um": grid_um}, **TWO_SPECIES, lateral_boundary=wall)
+def test_one_fp32_sweep_stays_within_fp32_rounding_of_the_oracle(wall, grid_um, gap_cm):
+    config = SimulationConfig(**{**CC, "grid_size_um": grid_um, "electrode_gap_cm": gap_cm}, **TWO_SPECIES,
+                              lateral_boundary=wall)
     rng = np.random.default_rng(19)
     shape = (config.no_xy, config.no_xy, config.no_z_with_buffer)
     pos = (rng.random(shape) * 1e7).astype(np.float32).astype(np.float64)
@@ -169,15 +175,19 @@ def test_one_fp32_sweep_stays_within_fp32_rounding_of_the_oracle(wall, grid_um):
     assert np.abs(gp[c] - ref_p[c]).max() <= 4e-7 * scale
     assert np.abs(gn[c] - ref_n[c]).max() <= 4e-7 * scale
     np.testing.assert_allclose(got3, ref3, rtol=1e-6)
+    # the k end planes stay exactly zero (windows store whole pairs there)
+    for a in (gp, gn):
+        assert not a[:, :, [0, -1]].any()
 
 
-@pytest.mark.parametrize("grid_um", GRIDS_UM)
-def test_fp32_one_and_two_values_per_lane_give_the_same_fields(grid_um):
-    """The two FP32 layouts (4-byte and 8-byte lanes over padded rows) run the
-    same voxel update: a whole run must agree bit for bit."""
+@pytest.mark.parametrize("grid_um,gap_cm", FP32_GRIDS)
+def test_fp32_one_and_two_values_per_lane_give_the_same_fields(grid_um, gap_cm):
+    """The two FP32 layouts (4-byte lanes in windows of 62 values, 8-byte lanes
+    in windows of 124 over padded rows) run the same voxel update: a whole run
+    must agree bit for bit, k end planes included."""
     from pulsed_ion_chamber_hip.pulses import draw_tracks
 
-    config = SimulationConfig(**{**CC, "grid_size_um": grid_um}, **TWO_SPECIES)
+    config = SimulationConfig(**{**CC, "grid_size_um": grid_um, "electrode_gap_cm": gap_cm}, **TWO_SPECIES)
     draw = draw_tracks(config)
     out = []
     for v in (1, 2):
