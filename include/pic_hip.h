@@ -131,8 +131,8 @@ int pic_reset(pic_sim *sim);
 int pic_synchronize(pic_sim *sim);
 
 /* Sweep launch shape, for tuning: rows per tile (1, 2, 4, 8), planes per
- * block chunk, and k values per lane (1, or 2 for FP32 fields);
- * 0 = automatic. Takes effect at the next launch. */
+ * block chunk, and k values per lane (1; 2 or 4 for FP32 fields, four with
+ * rows per tile 1 or 2 only); 0 = automatic. Takes effect at the next launch. */
 int pic_set_tiling(pic_sim *sim, int rows_per_tile, int planes_per_chunk, int values_per_lane);
 
 /* Kernel timing of the last pic_run_steps call when timing is enabled:

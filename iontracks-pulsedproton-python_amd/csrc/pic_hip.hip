@@ -88,7 +88,7 @@ struct pic_sim {
     int device = 0;
     hipStream_t stream = nullptr;
     size_t elem = 8;
-    int64_t pitch = 0;  // row pitch in elements (nzb, rounded up to even for FP32)
+    int64_t pitch = 0;  // row pitch in elements (nzb, rounded up to a multiple of four for FP32)
     int64_t n_vox = 0;  // allocated elements per array, no_xy^2 * pitch
     void *cur_p = nullptr, *cur_n = nullptr, *nxt_p = nullptr, *nxt_n = nullptr;
     double *dens = nullptr;        // [batch + 1 slots][no_xy^2]
@@ -111,7 +111,7 @@ struct pic_sim {
     double *xs = nullptr, *ys = nullptr;
     int64_t n_tracks = 0;
     std::vector<int64_t> offsets;
-    int force_tj = 0, force_chunk = 0, force_v1 = 0;
+    int force_tj = 0, force_chunk = 0, force_v = 0;
     int cus = 256;                 // compute units of the device (queried once)
     bool shape_valid = false;      // `shape` matches the tiling knobs
     SweepShape shape;
@@ -203,9 +203,23 @@ int sm_count(int device) {
     return v;
 }
 
-// k values per lane: FP32 fields take two (8-byte loads; the even row pitch
-// keeps every row 8-byte aligned)
-int values_per_lane(const pic_sim *s, bool force_v1) { return (s->prec == PIC_FP32 && !force_v1) ? 2 : 1; }
+// k values per lane: FP64 fields one (8-byte loads); FP32 fields one, two or
+// four (4-, 8- or 16-byte loads; the row pitch, a multiple of four values,
+// keeps every row 16-byte aligned). FP32 takes four on large grids -- where
+// four-value windows take no more lanes than two-value ones and 2-row tiles of
+// 16 planes give ~16 waves per CU -- and two otherwise. Measured on MI355X:
+// 1066^2 x 410, four values 1.68 ms against 1.83; 208^2 x 406 (a third of the
+// waves), 78.5 against 78.7 us; the 16^2..47^2 x 106..206 grids of a
+// saturation curve run one after another, 108 against 84 ms.
+int values_per_lane(const pic_sim *s, int force_v) {
+    if (s->prec == PIC_FP64) return 1;
+    if (force_v > 0) return force_v;
+    const int64_t n = s->p.nzb - 2, w4 = pic::window_outputs(4), w2 = pic::window_outputs(2);
+    const int64_t lateral = s->p.no_xy - 2;
+    const bool lanes_ok = ceil_div(n, w4) * w4 <= ceil_div(n, w2) * w2;
+    const bool waves_ok = ceil_div(lateral, 2) * ceil_div(n, w4) * ceil_div(lateral, 16) >= (int64_t)s->cus * 16;
+    return lanes_ok && waves_ok ? 4 : 2;
+}
 
 // Fixed tiling of one configuration: v values per lane, tj rows per tile,
 // `chunk` planes per block, wpb windows (waves) per block.
@@ -239,7 +253,7 @@ SweepShape sweep_shape(pic_sim *s) {
 SweepShape compute_sweep_shape(const pic_sim *s) {
     SweepShape sh;
     const int64_t no_xy = s->p.no_xy, nzb = s->p.nzb;
-    sh.v = values_per_lane(s, s->force_v1);
+    sh.v = values_per_lane(s, s->force_v);
     sh.n_windows = (int)ceil_div(nzb - 2, pic::window_outputs(sh.v));
     sh.wpb = std::min(sh.n_windows, 8);
     sh.n_kg = (int)ceil_div(sh.n_windows, sh.wpb);
@@ -253,11 +267,17 @@ SweepShape compute_sweep_shape(const pic_sim *s) {
     // FP64 154 us, against 162 for 2-row tiles of 8 planes) and on
     // 1066^2 x 410 (FP32 1.88 ms; FP64 3.65 ms, against 4.02 for 2-row tiles
     // of 8 planes). Small grids fall through to 1-row tiles of 4 planes.
-    const int64_t waves_wanted = (int64_t)cus * 8;
+    //
+    // Four values per lane: the 2-row tile (175-189 registers, two waves per
+    // SIMD), with chunks shortened first until there are ~16 waves per CU --
+    // two rounds of the 8 a CU holds (208^2 x 406: 8 planes, 78.5 us, against
+    // 84.6 for 16; 1066^2 x 410: 16 planes, 1.68 ms, against 1.73 for 8).
+    const int64_t waves_wanted = (int64_t)cus * (sh.v == 4 ? 16 : 8);
     int tj = s->force_tj;
     int chunk = s->force_chunk;
-    const int max_tj = (sh.v == 1 && s->prec == PIC_FP32) ? 8 : 4;
+    const int max_tj = (sh.v == 1 && s->prec == PIC_FP32) ? 8 : sh.v == 4 ? 2 : 4;
     const int chunk0 = 16;
+    if (tj <= 0 && sh.v == 4) tj = max_tj;
     if (tj <= 0) {
         const int c0 = chunk > 0 ? chunk : chunk0;
         tj = 1;
@@ -294,9 +314,14 @@ const void *sweep_fn_typed(int tj, bool dep, bool reflect) {
     switch (tj) {
         case 1: return sweep_fn_tj<S, A, 1, V>(dep, reflect);
         case 2: return sweep_fn_tj<S, A, 2, V>(dep, reflect);
-        case 4: return sweep_fn_tj<S, A, 4, V>(dep, reflect);
+        case 4:
+            if constexpr (V == 4) {
+                return nullptr;
+            } else {
+                return sweep_fn_tj<S, A, 4, V>(dep, reflect);
+            }
         case 8:
-            if constexpr (V == 2 || sizeof(S) == 8) {
+            if constexpr (V >= 2 || sizeof(S) == 8) {
                 return nullptr;
             } else {
                 return sweep_fn_tj<S, A, 8, V>(dep, reflect);
@@ -308,6 +333,7 @@ const void *sweep_fn_typed(int tj, bool dep, bool reflect) {
 // null for a combination that is not compiled (the tiling checks reject it first)
 const void *sweep_fn(int prec, int v, int tj, bool dep, bool reflect) {
     if (prec == PIC_FP64) return sweep_fn_typed<double, double, 1>(tj, dep, reflect);
+    if (v == 4) return sweep_fn_typed<float, float, 4>(tj, dep, reflect);
     return v == 2 ? sweep_fn_typed<float, float, 2>(tj, dep, reflect) : sweep_fn_typed<float, float, 1>(tj, dep, reflect);
 }
 
@@ -652,7 +678,7 @@ void swap_buffers(pic_sim *s) {
 // elements from one (i, j) row to the next: FP32 rows are padded to an even
 // length so that two-value lanes load 8-byte aligned pairs
 int64_t row_pitch(const pic_params *p, int precision) {
-    return precision == PIC_FP32 ? (p->nzb + 1) / 2 * 2 : p->nzb;
+    return precision == PIC_FP32 ? (p->nzb + 3) / 4 * 4 : p->nzb;
 }
 
 int check_params(const pic_params *p, int precision) {
@@ -695,7 +721,7 @@ size_t partial_cap_bytes(const pic_params *p) {
 // cost of the multi-configuration kernel, longer chunks to the march length.
 int batch_tiling(pic_batch *b) {
     int v = 2, wpb = 8;
-    for (pic_sim *s : b->sims) v = std::min(v, values_per_lane(s, false));
+    for (pic_sim *s : b->sims) v = std::min(v, values_per_lane(s, 0));
     for (pic_sim *s : b->sims)
         wpb = std::min(wpb, (int)ceil_div(s->p.nzb - 2, pic::window_outputs(v)));
     const int cus = b->sims[0]->cus;
@@ -1288,16 +1314,19 @@ int pic_synchronize(pic_sim *s) {
 int pic_set_tiling(pic_sim *s, int rows_per_tile, int planes_per_chunk, int values_per_lane) {
     if (!s) return fail(PIC_E_INVALID, "sim is NULL");
     if (rows_per_tile < 0 || planes_per_chunk < 0) return fail(PIC_E_INVALID, "negative tiling");
-    if (values_per_lane < 0 || values_per_lane > 2) return fail(PIC_E_INVALID, "values_per_lane must be 0, 1 or 2");
-    if (values_per_lane == 2 && s->prec != PIC_FP32)
-        return fail(PIC_E_INVALID, "two values per lane needs FP32 fields");
+    if (values_per_lane < 0 || values_per_lane > 4 || values_per_lane == 3)
+        return fail(PIC_E_INVALID, "values_per_lane must be 0, 1, 2 or 4");
+    if (values_per_lane >= 2 && s->prec != PIC_FP32)
+        return fail(PIC_E_INVALID, "%d values per lane needs FP32 fields", values_per_lane);
+    if (values_per_lane == 4 && rows_per_tile > 2)
+        return fail(PIC_E_INVALID, "four values per lane takes rows_per_tile 0, 1 or 2");
     if (rows_per_tile && rows_per_tile != 1 && rows_per_tile != 2 && rows_per_tile != 4 && rows_per_tile != 8)
         return fail(PIC_E_INVALID, "rows_per_tile must be 0, 1, 2, 4 or 8");
     if (rows_per_tile == 8 && (s->prec != PIC_FP32 || values_per_lane != 1))
         return fail(PIC_E_INVALID, "rows_per_tile 8 needs FP32 fields and values_per_lane 1");
     HIP_TRY(hipStreamSynchronize(s->stream));
-    const int old_tj = s->force_tj, old_chunk = s->force_chunk, old_v1 = s->force_v1;
-    s->force_v1 = values_per_lane == 1;
+    const int old_tj = s->force_tj, old_chunk = s->force_chunk, old_v = s->force_v;
+    s->force_v = values_per_lane;
     s->force_tj = rows_per_tile;
     s->force_chunk = planes_per_chunk;
     s->shape_valid = false;
@@ -1305,7 +1334,7 @@ int pic_set_tiling(pic_sim *s, int rows_per_tile, int planes_per_chunk, int valu
     if (rc) {
         s->force_tj = old_tj;
         s->force_chunk = old_chunk;
-        s->force_v1 = old_v1;
+        s->force_v = old_v;
         s->shape_valid = false;
     }
     return rc;

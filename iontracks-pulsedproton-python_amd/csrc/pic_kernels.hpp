@@ -17,10 +17,10 @@ namespace pic {
 constexpr int kWave = 64;
 constexpr int kOut = 62;  // output lanes per wave window (lanes 1..62)
 
-// k values a wave window writes: one value per lane, lanes 1..62; two values
-// per lane, the pairs of lanes 1..62 (windows step by 124 and overlap by four,
-// so every storing lane writes a whole 8-byte pair)
-constexpr int window_outputs(int v) { return v == 2 ? 2 * kWave - 4 : kWave - 2; }
+// k values a wave window of v values per lane writes: those of lanes 1..62,
+// so windows step by 62 v and overlap by 2 v, and every storing lane writes
+// all of its values at once (one 4-, 8- or 16-byte store)
+constexpr int window_outputs(int v) { return v * (kWave - 2); }
 constexpr int kXcds = 8;  // MI355X: blocks are dispatched round-robin over 8 XCDs
 
 // Map the hardware block index to a logical tile index so that each XCD
@@ -199,6 +199,8 @@ struct SweepArgs {
 
 typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
 // Fourth word of a raw buffer resource on gfx9-family devices (gfx950
 // included): 32-bit data format, no swizzle, no index stride.
@@ -261,7 +263,7 @@ constexpr int kBufferWord3 = 0x00020000;
 // this step deposits (a runtime flag in the multi-configuration kernel).
 template <typename S, typename A, int TJ, int V, bool REFLECT>
 __device__ __forceinline__ void sweep_body(const SweepArgs &a, const int logical, const bool DEP) {
-    static_assert(V == 1 || (V == 2 && sizeof(S) == 4), "two values per lane is the FP32 layout");
+    static_assert(V == 1 || ((V == 2 || V == 4) && sizeof(S) == 4), "two or four values per lane are FP32 layouts");
     constexpr int WIN = window_outputs(V);  // k step between windows
     const int kg = logical % a.n_kg;
     const int rest = logical / a.n_kg;
@@ -276,13 +278,13 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &a, const int logical
     if (wave < a.wpb && win < a.n_windows) {
         const int no_xy = a.no_xy, nzb = a.nzb, pitch = a.pitch;
         const int k0 = win * WIN + V * lane;
-        const bool k_in = k0 < nzb;  // with V = 2, k0 is even and k0+1 < pitch
+        const bool k_in = k0 < nzb;  // k0 is a multiple of V, and so is the pitch
         bool k_out[V], k_gap[V], k_score[V];
 #pragma unroll
         for (int v = 0; v < V; ++v) {
             const int q = V * lane + v, k = k0 + v;
-            // owned: lanes 1..62 (and k = 1 in the first window), inside the k range
-            const bool own_lane = V == 2 ? (q >= 2 || win == 0) && q <= 2 * kWave - 3 : q >= 1 && q <= kWave - 2;
+            // owned: lanes 1..62 (and k = 1 .. V-1 in the first window), inside the k range
+            const bool own_lane = (q >= V || (win == 0 && v >= 1)) && q < V * (kWave - 1);
             k_out[v] = own_lane && k >= 1 && k <= nzb - 2;
             k_gap[v] = k >= a.nze && k < a.nze + a.no_z;
             k_score[v] = k_out[v] && k_gap[v];
@@ -335,7 +337,15 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &a, const int logical
             const int sj = REFLECT ? min(max(j, 1), no_xy - 2) : j;
             const auto rp = plane_rsrc(P, si), rn = plane_rsrc(N, si);
             const int so = row_off(sj);
-            if constexpr (V == 2) {
+            if constexpr (V == 4) {
+                const f32x4 tp = __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(rp, vo_load, so, 0));
+                const f32x4 tn = __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(rn, vo_load, so, 0));
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    vp[v] = tp[v];
+                    vn[v] = tn[v];
+                }
+            } else if constexpr (V == 2) {
                 const f32x2 tp = __builtin_bit_cast(f32x2, __builtin_amdgcn_raw_buffer_load_b64(rp, vo_load, so, 0));
                 const f32x2 tn = __builtin_bit_cast(f32x2, __builtin_amdgcn_raw_buffer_load_b64(rn, vo_load, so, 0));
                 vp[0] = tp.x;
@@ -400,6 +410,21 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &a, const int logical
         auto store2 = [&](int i, int j, const S (&vp)[V], const S (&vn)[V]) {
             const auto rp = plane_rsrc(PO, i), rn = plane_rsrc(NO, i);
             const int so = row_off(j);
+            if constexpr (V == 4) {
+                // all four values: one the lane does not own is k = 0, a k end
+                // plane or the row padding, all of which stay zero
+                if (k_out[0] || k_out[1] || k_out[2] || k_out[3]) {
+                    f32x4 wp, wn;
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        wp[v] = k_out[v] ? vp[v] : 0.0f;
+                        wn[v] = k_out[v] ? vn[v] : 0.0f;
+                    }
+                    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, wp), rp, vo_store, so, 0);
+                    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, wn), rn, vo_store, so, 0);
+                }
+                return;
+            }
             if constexpr (V == 2) {
                 // the whole pair: a value the lane does not own is k = 0 or
                 // nzb-1, a k end plane, which must stay zero
@@ -468,10 +493,14 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &a, const int logical
         if (!REFLECT && i0 == 1) carry_ring_rows(0, bp[0], bn[0], true);
 
         // per-lane sums of the scored voxels; lanes whose k is never scored are
-        // masked once at the end (k_score is fixed for the whole march)
-        double sr[V], sp[V], sn[V];
+        // masked once at the end (k_score is fixed for the whole march). With
+        // four values per lane the lane's values are masked as they are added,
+        // into one sum per quantity, which saves 18 registers.
+        constexpr bool kMerged = V > 2;
+        constexpr int NS = kMerged ? 1 : V;
+        double sr[NS], sp[NS], sn[NS];
 #pragma unroll
-        for (int v = 0; v < V; ++v) sr[v] = sp[v] = sn[v] = 0.0;
+        for (int v = 0; v < NS; ++v) sr[v] = sp[v] = sn[v] = 0.0;
 
         auto plane_step = [&](int i, auto Oc) {
             constexpr int O = decltype(Oc)::value;  // (i - i0) % NB
@@ -493,15 +522,20 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &a, const int logical
                 const S(&cp)[TJ + 2][V] = bp[CB];
                 const S(&cn)[TJ + 2][V] = bn[CB];
                 S p_km[V], p_kp[V], n_km[V], n_kp[V];
-                if constexpr (V == 2) {
-                    p_km[0] = __shfl_up(cp[r + 1][1], 1, kWave);
-                    p_kp[0] = cp[r + 1][1];
-                    p_km[1] = cp[r + 1][0];
-                    p_kp[1] = __shfl_down(cp[r + 1][0], 1, kWave);
-                    n_km[0] = __shfl_up(cn[r + 1][1], 1, kWave);
-                    n_kp[0] = cn[r + 1][1];
-                    n_km[1] = cn[r + 1][0];
-                    n_kp[1] = __shfl_down(cn[r + 1][0], 1, kWave);
+                if constexpr (V >= 2) {
+                    // inner neighbours are the lane's own values; the outer
+                    // ones come from the adjacent lanes
+                    p_km[0] = __shfl_up(cp[r + 1][V - 1], 1, kWave);
+                    p_kp[V - 1] = __shfl_down(cp[r + 1][0], 1, kWave);
+                    n_km[0] = __shfl_up(cn[r + 1][V - 1], 1, kWave);
+                    n_kp[V - 1] = __shfl_down(cn[r + 1][0], 1, kWave);
+#pragma unroll
+                    for (int v = 1; v < V; ++v) {
+                        p_km[v] = cp[r + 1][v - 1];
+                        p_kp[v - 1] = cp[r + 1][v];
+                        n_km[v] = cn[r + 1][v - 1];
+                        n_kp[v - 1] = cn[r + 1][v];
+                    }
                 } else {
                     p_km[0] = __shfl_up(cp[r + 1][0], 1, kWave);
                     p_kp[0] = __shfl_down(cp[r + 1][0], 1, kWave);
@@ -553,11 +587,24 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &a, const int logical
                         }
                     }
                     if (inside) {
+                        if constexpr (kMerged) {
+                            double r = 0.0, pp = 0.0, nn = 0.0;
 #pragma unroll
-                        for (int v = 0; v < V; ++v) {
-                            sr[v] += rec[v];
-                            sp[v] += (double)po[v];
-                            sn[v] += (double)no[v];
+                            for (int v = 0; v < V; ++v) {
+                                r += k_score[v] ? rec[v] : 0.0;
+                                pp += k_score[v] ? (double)po[v] : 0.0;
+                                nn += k_score[v] ? (double)no[v] : 0.0;
+                            }
+                            sr[0] += r;
+                            sp[0] += pp;
+                            sn[0] += nn;
+                        } else {
+#pragma unroll
+                            for (int v = 0; v < V; ++v) {
+                                sr[v] += rec[v];
+                                sp[v] += (double)po[v];
+                                sn[v] += (double)no[v];
+                            }
                         }
                     }
                     store2(i, j, po, no);
@@ -567,11 +614,17 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &a, const int logical
         for (int i = i0; i < i1; i += NB) {
             static_for<NB>([&](auto Oc) { plane_step(i + decltype(Oc)::value, Oc); });
         }
+        if constexpr (kMerged) {
+            acc_r += sr[0];
+            acc_p += sp[0];
+            acc_n += sn[0];
+        } else {
 #pragma unroll
-        for (int v = 0; v < V; ++v) {
-            acc_r += k_score[v] ? sr[v] : 0.0;
-            acc_p += k_score[v] ? sp[v] : 0.0;
-            acc_n += k_score[v] ? sn[v] : 0.0;
+            for (int v = 0; v < V; ++v) {
+                acc_r += k_score[v] ? sr[v] : 0.0;
+                acc_p += k_score[v] ? sp[v] : 0.0;
+                acc_n += k_score[v] ? sn[v] : 0.0;
+            }
         }
     }
     block_sum3(acc_r, acc_p, acc_n);

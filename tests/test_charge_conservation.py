@@ -181,21 +181,23 @@ def test_one_fp32_sweep_stays_within_fp32_rounding_of_the_oracle(wall, grid_um, 
 
 
 @pytest.mark.parametrize("grid_um,gap_cm", FP32_GRIDS)
-def test_fp32_one_and_two_values_per_lane_give_the_same_fields(grid_um, gap_cm):
-    """The two FP32 layouts (4-byte lanes in windows of 62 values, 8-byte lanes
-    in windows of 124 over padded rows) run the same voxel update: a whole run
-    must agree bit for bit, k end planes included."""
+def test_fp32_one_two_and_four_values_per_lane_give_the_same_fields(grid_um, gap_cm):
+    """The FP32 layouts (4-, 8- and 16-byte lanes in windows of 62, 124 and
+    248 values over padded rows) run the same voxel update: a whole run must
+    agree bit for bit, k end planes included."""
     from pulsed_ion_chamber_hip.pulses import draw_tracks
 
     config = SimulationConfig(**{**CC, "grid_size_um": grid_um, "electrode_gap_cm": gap_cm}, **TWO_SPECIES)
     draw = draw_tracks(config)
     out = []
-    for v in (1, 2):
+    for v in (1, 2, 4):
         with HipSimulation(config, "fp32") as sim:
             sim.set_tiling(values_per_lane=v)
             sim.upload_tracks(*draw)
             sim.run_steps(0, config.total_time_steps)
             out.append((sim.read_per_step(), *sim.read_fields()))
-    np.testing.assert_array_equal(out[0][1], out[1][1])
-    np.testing.assert_array_equal(out[0][2], out[1][2])
-    np.testing.assert_allclose(out[0][0], out[1][0], rtol=1e-12, atol=1e-300)
+    for other in out[1:]:
+        np.testing.assert_array_equal(out[0][1], other[1])
+        np.testing.assert_array_equal(out[0][2], other[2])
+        # the scored sums add the same terms in another order
+        np.testing.assert_allclose(out[0][0], other[0], rtol=1e-12, atol=1e-300)
