@@ -53,7 +53,7 @@ int fail(int code, const char *fmt, ...) {
 int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 struct SweepShape {
-    int tj = 0, chunk = 0, n_windows = 0, wpb = 0, n_kg = 0, n_jt = 0, n_ic = 0, v = 1;
+    int tj = 0, chunk = 0, n_windows = 0, wpb = 0, n_kg = 0, n_jt = 0, n_ic = 0, v = 1, win_step = 0;
     int64_t blocks() const { return (int64_t)n_kg * n_jt * n_ic; }
 };
 
@@ -88,7 +88,7 @@ struct pic_sim {
     int device = 0;
     hipStream_t stream = nullptr;
     size_t elem = 8;
-    int64_t pitch = 0;  // row pitch in elements (nzb, rounded up to a multiple of four for FP32)
+    int64_t pitch = 0;  // row pitch in elements (nzb; FP32 rounded up to even, or to a multiple of four)
     int64_t n_vox = 0;  // allocated elements per array, no_xy^2 * pitch
     void *cur_p = nullptr, *cur_n = nullptr, *nxt_p = nullptr, *nxt_n = nullptr;
     double *dens = nullptr;        // [batch + 1 slots][no_xy^2]
@@ -204,21 +204,28 @@ int sm_count(int device) {
 }
 
 // k values per lane: FP64 fields one (8-byte loads); FP32 fields one, two or
-// four (4-, 8- or 16-byte loads; the row pitch, a multiple of four values,
-// keeps every row 16-byte aligned). FP32 takes four on large grids -- where
-// four-value windows take no more lanes than two-value ones and 2-row tiles of
-// 16 planes give ~16 waves per CU -- and two otherwise. Measured on MI355X:
-// 1066^2 x 410, four values 1.68 ms against 1.83; 208^2 x 406 (a third of the
-// waves), 78.5 against 78.7 us; the 16^2..47^2 x 106..206 grids of a
-// saturation curve run one after another, 108 against 84 ms.
+// four (4-, 8- or 16-byte loads). FP32 grids take four when four-value windows
+// take no more lanes than two-value ones and 2-row tiles of 16 planes give ~16
+// waves per CU of MI355X (256 CUs; fixed, since the row pitch follows from
+// this choice before any device is opened), and two otherwise. Measured on
+// MI355X: 1066^2 x 410, four values 1.68 ms against 1.74 (each at its best
+// pitch); 208^2 x 406 (a third of the waves), 78.5 against 78.7 us at the
+// same pitch; the 16^2..47^2 x 106..206 grids of a saturation curve run one
+// after another, 108 against 84 ms; 2656^2 x 1010 (nine two-value windows,
+// five four-value ones), 30.2 against 26.9 ms.
+bool four_values_suit(const pic_params *p) {
+    constexpr int64_t kCus = 256;
+    const int64_t n = p->nzb - 2, w4 = pic::window_outputs(4), w2 = pic::window_step(2, n);
+    const int64_t lateral = p->no_xy - 2;
+    const bool lanes_ok = ceil_div(n, w4) * w4 <= ceil_div(n, w2) * w2;
+    const bool waves_ok = ceil_div(lateral, 2) * ceil_div(n, w4) * ceil_div(lateral, 16) >= kCus * 16;
+    return lanes_ok && waves_ok;
+}
+
 int values_per_lane(const pic_sim *s, int force_v) {
     if (s->prec == PIC_FP64) return 1;
     if (force_v > 0) return force_v;
-    const int64_t n = s->p.nzb - 2, w4 = pic::window_outputs(4), w2 = pic::window_outputs(2);
-    const int64_t lateral = s->p.no_xy - 2;
-    const bool lanes_ok = ceil_div(n, w4) * w4 <= ceil_div(n, w2) * w2;
-    const bool waves_ok = ceil_div(lateral, 2) * ceil_div(n, w4) * ceil_div(lateral, 16) >= (int64_t)s->cus * 16;
-    return lanes_ok && waves_ok ? 4 : 2;
+    return four_values_suit(&s->p) ? 4 : 2;
 }
 
 // Fixed tiling of one configuration: v values per lane, tj rows per tile,
@@ -226,7 +233,8 @@ int values_per_lane(const pic_sim *s, int force_v) {
 SweepShape fixed_shape(const pic_sim *s, int v, int tj, int chunk, int wpb) {
     SweepShape sh;
     sh.v = v;
-    sh.n_windows = (int)ceil_div(s->p.nzb - 2, pic::window_outputs(v));
+    sh.win_step = pic::window_step(v, s->p.nzb - 2);
+    sh.n_windows = (int)ceil_div(s->p.nzb - 2, sh.win_step);
     sh.wpb = wpb;
     sh.n_kg = (int)ceil_div(sh.n_windows, wpb);
     const int64_t rows = s->p.no_xy - 2, planes = s->p.no_xy - 2;
@@ -254,7 +262,8 @@ SweepShape compute_sweep_shape(const pic_sim *s) {
     SweepShape sh;
     const int64_t no_xy = s->p.no_xy, nzb = s->p.nzb;
     sh.v = values_per_lane(s, s->force_v);
-    sh.n_windows = (int)ceil_div(nzb - 2, pic::window_outputs(sh.v));
+    sh.win_step = pic::window_step(sh.v, nzb - 2);
+    sh.n_windows = (int)ceil_div(nzb - 2, sh.win_step);
     sh.wpb = std::min(sh.n_windows, 8);
     sh.n_kg = (int)ceil_div(sh.n_windows, sh.wpb);
     const int64_t rows = no_xy - 2, planes = no_xy - 2;
@@ -580,6 +589,7 @@ int make_sweep_args(pic_sim *s, const SweepShape &sh, int64_t step, bool dep, co
     a.nze = (int)s->p.nze;
     a.no_z = (int)s->p.no_z;
     a.n_windows = sh.n_windows;
+    a.win_step = sh.win_step;
     a.wpb = sh.wpb;
     a.n_kg = sh.n_kg;
     a.n_jt = sh.n_jt;
@@ -677,8 +687,17 @@ void swap_buffers(pic_sim *s) {
 
 // elements from one (i, j) row to the next: FP32 rows are padded to an even
 // length so that two-value lanes load 8-byte aligned pairs
+// FP32 rows are padded so every row is aligned for the layout's loads: to a
+// multiple of four values (16 bytes) where the grid takes four values per
+// lane, else to an even length (8 bytes). The pitch also sets how rows and
+// planes fall on HBM channels, and rows padded further than needed measured
+// slower: 1066^2 x 410 with two values per lane 1.74 ms at pitch 410, 1.83 at
+// 412 and 416, 1.98 at 448; 2656^2 x 1010, 26.9 ms at 1010 against 40.8 at
+// 1012.
 int64_t row_pitch(const pic_params *p, int precision) {
-    return precision == PIC_FP32 ? (p->nzb + 3) / 4 * 4 : p->nzb;
+    if (precision != PIC_FP32) return p->nzb;
+    const int64_t m = four_values_suit(p) ? 4 : 2;
+    return (p->nzb + m - 1) / m * m;
 }
 
 int check_params(const pic_params *p, int precision) {
@@ -723,7 +742,7 @@ int batch_tiling(pic_batch *b) {
     int v = 2, wpb = 8;
     for (pic_sim *s : b->sims) v = std::min(v, values_per_lane(s, 0));
     for (pic_sim *s : b->sims)
-        wpb = std::min(wpb, (int)ceil_div(s->p.nzb - 2, pic::window_outputs(v)));
+        wpb = std::min(wpb, (int)ceil_div(s->p.nzb - 2, pic::window_step(v, s->p.nzb - 2)));
     const int cus = b->sims[0]->cus;
     std::vector<std::pair<int, int>> cands;
     if (b->force_tj > 0 || b->force_chunk > 0)
@@ -1320,6 +1339,9 @@ int pic_set_tiling(pic_sim *s, int rows_per_tile, int planes_per_chunk, int valu
         return fail(PIC_E_INVALID, "%d values per lane needs FP32 fields", values_per_lane);
     if (values_per_lane == 4 && rows_per_tile > 2)
         return fail(PIC_E_INVALID, "four values per lane takes rows_per_tile 0, 1 or 2");
+    if (values_per_lane == 4 && s->pitch % 4 != 0)
+        return fail(PIC_E_INVALID, "four values per lane needs rows padded to a multiple of four values; this grid's rows "
+                    "hold %lld (four values are used by default where they pay)", (long long)s->pitch);
     if (rows_per_tile && rows_per_tile != 1 && rows_per_tile != 2 && rows_per_tile != 4 && rows_per_tile != 8)
         return fail(PIC_E_INVALID, "rows_per_tile must be 0, 1, 2, 4 or 8");
     if (rows_per_tile == 8 && (s->prec != PIC_FP32 || values_per_lane != 1))

@@ -2,8 +2,10 @@
 //
 // Grid layout: each species is (no_xy, no_xy, nzb), k fastest, stored as S
 // (double or float) with a row pitch of `pitch` >= nzb elements (FP32 rows are
-// padded to an even length so every row starts 8-byte aligned; the padding is
-// zero and never written). Interior voxels are 1 <= i,j <= no_xy-2,
+// padded to an even length, or to a multiple of four where the sweep takes
+// four values per lane, so every row is aligned for the lane's loads; the
+// padding is zero and only ever written with zeros). Interior voxels are
+// 1 <= i,j <= no_xy-2,
 // 1 <= k <= nzb-2.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -21,6 +23,16 @@ constexpr int kOut = 62;  // output lanes per wave window (lanes 1..62)
 // so windows step by 62 v and overlap by 2 v, and every storing lane writes
 // all of its values at once (one 4-, 8- or 16-byte store)
 constexpr int window_outputs(int v) { return v * (kWave - 2); }
+
+// k step between the windows of a row of n interior values: window_outputs,
+// or for two values per lane 126 where that takes fewer windows (lane 0's
+// second and lane 63's first value are then written too, by single-value
+// stores; 1008 values are eight such windows but nine of 124)
+__host__ __device__ inline int window_step(int v, long long n) {
+    const int narrow = window_outputs(v), wide = 2 * kWave - 2;
+    if (v != 2) return narrow;
+    return (n + wide - 1) / wide < (n + narrow - 1) / narrow ? wide : narrow;
+}
 constexpr int kXcds = 8;  // MI355X: blocks are dispatched round-robin over 8 XCDs
 
 // Map the hardware block index to a logical tile index so that each XCD
@@ -187,7 +199,8 @@ struct SweepArgs {
     double *prev_out4;            // (injected, recombined, total+, total-), or null
     int no_xy, nzb, nze, no_z;
     int pitch;      // elements from one (i, j) row to the next
-    int n_windows;  // k windows of window_outputs(V) outputs covering k = 1..nzb-2
+    int n_windows;  // k windows covering k = 1..nzb-2
+    int win_step;   // k values between windows (window_step)
     int wpb;        // windows (waves) per block
     int n_kg, n_jt, n_ic;
     int chunk;  // planes per block
@@ -277,17 +290,29 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &a, const int logical
     double acc_r = 0.0, acc_p = 0.0, acc_n = 0.0;
     if (wave < a.wpb && win < a.n_windows) {
         const int no_xy = a.no_xy, nzb = a.nzb, pitch = a.pitch;
-        const int k0 = win * WIN + V * lane;
+        const int step = V == 2 ? a.win_step : WIN;
+        const int k0 = win * step + V * lane;
         const bool k_in = k0 < nzb;  // k0 is a multiple of V, and so is the pitch
         bool k_out[V], k_gap[V], k_score[V];
 #pragma unroll
         for (int v = 0; v < V; ++v) {
             const int q = V * lane + v, k = k0 + v;
-            // owned: lanes 1..62 (and k = 1 .. V-1 in the first window), inside the k range
-            const bool own_lane = (q >= V || (win == 0 && v >= 1)) && q < V * (kWave - 1);
+            // owned: lanes 1..62 (and k = 1 .. V-1 in the first window), inside
+            // the k range; 126-value windows also own q = 1 and q = 126
+            const bool own_lane = step == 2 * kWave - 2 ? q >= 1 && q <= 2 * kWave - 2
+                                                        : (q >= V || (win == 0 && v >= 1)) && q < V * (kWave - 1);
             k_out[v] = own_lane && k >= 1 && k <= nzb - 2;
             k_gap[v] = k >= a.nze && k < a.nze + a.no_z;
             k_score[v] = k_out[v] && k_gap[v];
+        }
+        // two values per lane: a lane writes its pair in one store when the
+        // value it does not own (if any) is a k end plane; only the edge lanes
+        // of 126-value windows, whose other value the neighbour window owns,
+        // write single values
+        bool pair_store = false;
+        if constexpr (V == 2) {
+            const bool z0 = k0 == 0 || k0 == nzb - 1, z1 = k0 + 1 == nzb - 1;
+            pair_store = (k_out[0] || k_out[1]) && (k_out[0] || z0) && (k_out[1] || z1);
         }
         const int j0 = 1 + jt * TJ;
         const int nrows = min(TJ, no_xy - 1 - j0);  // valid own rows
@@ -426,15 +451,16 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &a, const int logical
                 return;
             }
             if constexpr (V == 2) {
-                // the whole pair: a value the lane does not own is k = 0 or
-                // nzb-1, a k end plane, which must stay zero
-                if (k_out[0] || k_out[1]) {
+                // the whole pair where the value the lane does not own is k = 0
+                // or nzb-1, a k end plane, which must stay zero; single values
+                // only at the edges of 126-value windows
+                if (pair_store) {
                     const f32x2 wp = {k_out[0] ? vp[0] : 0.0f, k_out[1] ? vp[1] : 0.0f};
                     const f32x2 wn = {k_out[0] ? vn[0] : 0.0f, k_out[1] ? vn[1] : 0.0f};
                     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, wp), rp, vo_store, so, 0);
                     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, wn), rn, vo_store, so, 0);
+                    return;
                 }
-                return;
             }
 #pragma unroll
             for (int v = 0; v < V; ++v)

@@ -57,8 +57,8 @@ def test_mem_required_counts_four_carrier_arrays():
     size32 = ctypes.c_size_t()
     assert lib.pic_mem_required(ctypes.byref(p), _lib.PIC_FP64, ctypes.byref(size64)) == 0
     assert lib.pic_mem_required(ctypes.byref(p), _lib.PIC_FP32, ctypes.byref(size32)) == 0
-    # FP32 rows are padded to a multiple of four values: 50 -> 52
-    assert size64.value - size32.value == 4 * 100 * 100 * (50 * 8 - 52 * 4)
+    # 50 values per row: an even length, so FP32 rows need no padding
+    assert size64.value - size32.value == 4 * 100 * 100 * 50 * 4
 
 
 def test_no_device_raises_runtime_error_not_a_fallback():
