@@ -271,11 +271,14 @@ SweepShape compute_sweep_shape(const pic_sim *s) {
     // Enough waves for ~8 per CU, with the tallest tile (fewest halo rows) and
     // the longest chunk (fewest prologue planes) that still get there.
     // Measured on MI355X over whole runs (depositing steps included), with the
-    // buffer-addressed sweep: 4-row tiles of 16 planes are best for both
-    // precisions on 208^2 x 406 (FP32 80.5 us, against 84.4 for 2-row tiles;
-    // FP64 154 us, against 162 for 2-row tiles of 8 planes) and on
-    // 1066^2 x 410 (FP32 1.88 ms; FP64 3.65 ms, against 4.02 for 2-row tiles
-    // of 8 planes). Small grids fall through to 1-row tiles of 4 planes.
+    // buffer-addressed sweep: 4-row tiles of 16 planes are best for one and two
+    // values per lane on 208^2 x 406 (FP32 80.5 us, against 84.4 for 2-row
+    // tiles; FP64 154 us, against 162 for 2-row tiles of 8 planes) and on
+    // 1066^2 x 410 (FP32 two values 1.88 ms; FP64 3.65 ms, against 4.02 for
+    // 2-row tiles of 8 planes), before whole-lane stores took 4 to 8 % off the
+    // FP32 figures; 12- and 24-plane chunks tie with 16 on 208^2 x 406, 14,
+    // 18 and 20 lose 10-13 % (the busiest CU marches more block-planes). Small
+    // grids fall through to 1-row tiles of 4 planes.
     //
     // Four values per lane: the 2-row tile (175-189 registers, two waves per
     // SIMD), with chunks shortened first until there are ~16 waves per CU --
