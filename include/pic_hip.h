@@ -135,6 +135,11 @@ int pic_synchronize(pic_sim *sim);
  * rows per tile 1 or 2 only); 0 = automatic. Takes effect at the next launch. */
 int pic_set_tiling(pic_sim *sim, int rows_per_tile, int planes_per_chunk, int values_per_lane);
 
+/* The tiling the next sweep launch uses: the rule's choice for every knob
+ * pic_set_tiling left at 0 (replaces reading back the threads per block the
+ * reference's set_threads_per_block stores, solver_cuda.py:121-132). */
+int pic_get_tiling(pic_sim *sim, int *rows_per_tile, int *planes_per_chunk, int *values_per_lane);
+
 /* Kernel timing of the last pic_run_steps call when timing is enabled:
  * summed milliseconds of the sweep and of the deposition kernels (with their
  * launch counts) and of the ring + reduce kernels, measured with HIP events

@@ -225,7 +225,8 @@ bool four_values_suit(const pic_params *p) {
 int values_per_lane(const pic_sim *s, int force_v) {
     if (s->prec == PIC_FP64) return 1;
     if (force_v > 0) return force_v;
-    return four_values_suit(&s->p) ? 4 : 2;
+    if (s->force_tj > 2) return 2;  // tiles of four rows exist for one and two values
+    return four_values_suit(&s->p) && s->pitch % 4 == 0 ? 4 : 2;
 }
 
 // Fixed tiling of one configuration: v values per lane, tj rows per tile,
@@ -1363,6 +1364,15 @@ int pic_set_tiling(pic_sim *s, int rows_per_tile, int planes_per_chunk, int valu
         s->shape_valid = false;
     }
     return rc;
+}
+
+int pic_get_tiling(pic_sim *s, int *rows_per_tile, int *planes_per_chunk, int *values_per_lane) {
+    if (!s) return fail(PIC_E_INVALID, "sim is NULL");
+    const SweepShape sh = sweep_shape(s);
+    if (rows_per_tile) *rows_per_tile = sh.tj;
+    if (planes_per_chunk) *planes_per_chunk = sh.chunk;
+    if (values_per_lane) *values_per_lane = sh.v;
+    return PIC_OK;
 }
 
 int pic_enable_timing(pic_sim *s, int enable) {

@@ -47,6 +47,7 @@ EXPORTED = (
     "pic_reset",
     "pic_synchronize",
     "pic_set_tiling",
+    "pic_get_tiling",
     "pic_enable_timing",
     "pic_get_timing",
     "pic_set_graphs",
@@ -135,6 +136,7 @@ def open_library():
         "pic_reset": [_VP],
         "pic_synchronize": [_VP],
         "pic_set_tiling": [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int],
+        "pic_get_tiling": [_VP] + [ctypes.POINTER(ctypes.c_int)] * 3,
         "pic_enable_timing": [_VP, ctypes.c_int],
         "pic_get_timing": [_VP, _DP, _IP, _DP, _IP, _DP],
         "pic_set_graphs": [_VP, ctypes.c_int],

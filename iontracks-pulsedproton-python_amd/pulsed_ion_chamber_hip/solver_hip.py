@@ -134,6 +134,13 @@ class HipSimulation:
         _lib.check(self.L.pic_set_tiling(self.handle, int(rows_per_tile), int(planes_per_chunk), int(values_per_lane)),
                    "pic_set_tiling")
 
+    def get_tiling(self) -> dict:
+        """The tiling the next sweep launch uses (the library's choice for every
+        knob set_tiling left at 0)."""
+        v = [ctypes.c_int() for _ in range(3)]
+        _lib.check(self.L.pic_get_tiling(self.handle, *[ctypes.byref(x) for x in v]), "pic_get_tiling")
+        return dict(rows_per_tile=v[0].value, planes_per_chunk=v[1].value, values_per_lane=v[2].value)
+
     def enable_timing(self, on: bool = True) -> None:
         _lib.check(self.L.pic_enable_timing(self.handle, 1 if on else 0), "pic_enable_timing")
 

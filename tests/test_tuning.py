@@ -41,3 +41,38 @@ def test_explicit_tiling_and_bad_values():
         run_simulation_hip(cfg, progress=False, tiling="fast")
     with pytest.raises(ValueError):
         run_simulation_hip(cfg, progress=False, tiling=(3, 4, 1))
+
+
+def test_default_tiling_follows_the_grid():
+    """Four values per lane (2-row tiles) only on grids with enough waves for
+    them; automatic values fall back to two under forced 4-row tiles; the
+    default four-value run matches two values per lane bit for bit."""
+    from configs import readme_config
+    from pulsed_ion_chamber_hip.pulses import draw_tracks
+    from pulsed_ion_chamber_hip.solver_hip import HipSimulation
+
+    small = readme_config(16.0, 0.0077)
+    large = readme_config(5.0, 0.065)  # 268^2 x 406
+    with HipSimulation(small, "fp32") as sim:
+        assert sim.get_tiling()["values_per_lane"] == 2
+    with HipSimulation(large, "fp64") as sim:
+        assert sim.get_tiling()["values_per_lane"] == 1
+    draw = draw_tracks(large)
+    out = []
+    with HipSimulation(large, "fp32") as sim:
+        tiling = sim.get_tiling()
+        assert (tiling["rows_per_tile"], tiling["values_per_lane"]) == (2, 4)
+        sim.set_tiling(4, 16, 0)
+        assert sim.get_tiling() == dict(rows_per_tile=4, planes_per_chunk=16, values_per_lane=2)
+        sim.set_tiling(0, 0, 0)
+        assert sim.get_tiling() == tiling
+        sim.upload_tracks(*draw)
+        for t in ((0, 0, 0), (4, 16, 2)):
+            sim.set_tiling(*t)
+            sim.reset()
+            sim.run_steps(0, 30)
+            out.append((sim.read_per_step(0, 30), *sim.read_fields()))
+    np.testing.assert_array_equal(out[0][1], out[1][1])
+    np.testing.assert_array_equal(out[0][2], out[1][2])
+    np.testing.assert_allclose(out[0][0], out[1][0], rtol=1e-12, atol=1e-300)
+    assert out[0][0][:, 0].sum() > 0  # the steps deposited
