@@ -143,7 +143,12 @@ int pic_get_tiling(pic_sim *sim, int *rows_per_tile, int *planes_per_chunk, int 
 /* Kernel timing of the last pic_run_steps call when timing is enabled:
  * summed milliseconds of the sweep and of the deposition kernels (with their
  * launch counts) and of the ring + reduce kernels, measured with HIP events
- * recorded around each launch on the handle's stream. */
+ * recorded on the handle's stream. enable = 0 turns it off, 1 brackets each
+ * launch, n > 1 brackets runs of up to n back-to-back sweep launches with one
+ * pair (and each other launch): a pair costs the stream a few microseconds,
+ * so grouping keeps a timed run as fast as an untimed one while every sweep
+ * is still timed (its time then includes the ~1 us between launches). The
+ * launch counts are those of the launches bracketed. */
 int pic_enable_timing(pic_sim *sim, int enable);
 int pic_get_timing(pic_sim *sim, double *sweep_ms, int64_t *sweep_launches, double *deposit_ms,
                    int64_t *deposit_launches, double *other_ms);

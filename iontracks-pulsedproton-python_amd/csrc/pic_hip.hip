@@ -116,8 +116,11 @@ struct pic_sim {
     bool shape_valid = false;      // `shape` matches the tiling knobs
     SweepShape shape;
     bool timing = false;
+    int timing_group = 1;  // sweep launches per event pair
+    int group_open = 0;    // sweeps launched since the open pair's first event
     std::vector<hipEvent_t> ev_pool;  // reused across calls
     std::vector<int> ev_kind;         // per recorded event: 0 sweep, 1 deposit, 2 ring + reduce
+    std::vector<int> ev_count;        // per pair (at its first event): launches it brackets
     size_t ev_used = 0;
     bool in_run = false;  // events are recorded inside pic_run_steps only
     bool graphs = false;  // replay a run's launches through graph chains (never while timing)
@@ -502,8 +505,9 @@ pic::BatchTable<D> batch_table(const D *descs, const int *offs, int n_cfg) {
     return t;
 }
 
-// With timing on, every kernel group is bracketed by a pair of events on the
-// handle's stream; pic_get_timing sums the pairs per kind.
+// With timing on, launches are bracketed by pairs of events on the handle's
+// stream: each launch, or runs of up to timing_group back-to-back sweep
+// launches; pic_get_timing sums the pairs per kind.
 void record(pic_sim *s, int kind) {
     if (!s->timing || !s->in_run) return;
     if (s->ev_used == s->ev_pool.size()) {
@@ -513,9 +517,19 @@ void record(pic_sim *s, int kind) {
         if (hipEventCreateWithFlags(&e, hipEventReleaseToDevice) != hipSuccess) return;
         s->ev_pool.push_back(e);
         s->ev_kind.push_back(kind);
+        s->ev_count.push_back(1);
     }
     s->ev_kind[s->ev_used] = kind;
+    s->ev_count[s->ev_used] = 1;
     if (hipEventRecord(s->ev_pool[s->ev_used], s->stream) == hipSuccess) ++s->ev_used;
+}
+
+// end the open run of timed sweeps, if any
+void close_group(pic_sim *s) {
+    if (!s->group_open) return;
+    record(s, 0);
+    if (s->ev_used >= 2) s->ev_count[s->ev_used - 2] = s->group_open;
+    s->group_open = 0;
 }
 
 void record_batch(pic_batch *b, int kind) {
@@ -537,6 +551,14 @@ int emit(pic_sim *s, const KLaunch &l, hipStream_t st) {
         s->sink->push_back(l);
         return PIC_OK;
     }
+    if (s->timing && s->in_run && l.kind == 0 && s->timing_group > 1) {
+        if (!s->group_open) record(s, 0);
+        const int rc = launch_now(l, st);
+        if (rc) return rc;
+        if (++s->group_open == s->timing_group) close_group(s);
+        return PIC_OK;
+    }
+    if (s->timing && s->in_run) close_group(s);  // another kind of launch ends a run of sweeps
     record(s, l.kind);
     const int rc = launch_now(l, st);
     if (rc) return rc;
@@ -630,6 +652,7 @@ int enqueue_sweep(pic_sim *s, int64_t step, bool dep, const double *dens, const 
 // fold the pending step's partials into its record (end of a run of steps)
 int flush_pending(pic_sim *s, hipStream_t st = nullptr) {
     if (!s->pending.out4) return PIC_OK;
+    close_group(s);
     record(s, 2);
     hipLaunchKernelGGL(pic::reduce_kernel, dim3(1), dim3(256), 0, st ? st : s->stream, s->pending.part, s->pending.n,
                        s->pending.dep, s->pending.n_dep, s->pending.out4);
@@ -1160,6 +1183,7 @@ int pic_run_steps(pic_sim *s, int64_t first, int64_t count) {
                     (long long)(first + count), (long long)s->n_steps);
     HIP_TRY(hipStreamSynchronize(s->stream));  // events of an earlier call may still be pending
     s->ev_used = 0;
+    s->group_open = 0;
     s->in_run = true;
     int rc = PIC_OK;
     const int n_tiles = (int)gather_blocks(s);
@@ -1186,6 +1210,7 @@ int pic_run_steps(pic_sim *s, int64_t first, int64_t count) {
         if (chained && !rc) rc = submit(s->seg, s->device, s->stream);
     }
     if (!rc) rc = flush_pending(s);
+    close_group(s);
     s->in_run = false;
     return rc;
 }
@@ -1377,7 +1402,9 @@ int pic_get_tiling(pic_sim *s, int *rows_per_tile, int *planes_per_chunk, int *v
 
 int pic_enable_timing(pic_sim *s, int enable) {
     if (!s) return fail(PIC_E_INVALID, "sim is NULL");
+    if (enable < 0) return fail(PIC_E_INVALID, "enable must be 0, 1 or a number of sweep launches per event pair");
     s->timing = enable != 0;
+    s->timing_group = enable > 1 ? enable : 1;
     return PIC_OK;
 }
 
@@ -1398,7 +1425,7 @@ int pic_get_timing(pic_sim *s, double *sweep_ms, int64_t *sweep_launches, double
         HIP_TRY(hipEventElapsedTime(&ms, s->ev_pool[q], s->ev_pool[q + 1]));
         const int kind = s->ev_kind[q];
         t[kind] += ms;
-        ++n[kind];
+        n[kind] += s->ev_count[q];
     }
     if (sweep_ms) *sweep_ms = t[0];
     if (sweep_launches) *sweep_launches = n[0];

@@ -141,8 +141,11 @@ class HipSimulation:
         _lib.check(self.L.pic_get_tiling(self.handle, *[ctypes.byref(x) for x in v]), "pic_get_tiling")
         return dict(rows_per_tile=v[0].value, planes_per_chunk=v[1].value, values_per_lane=v[2].value)
 
-    def enable_timing(self, on: bool = True) -> None:
-        _lib.check(self.L.pic_enable_timing(self.handle, 1 if on else 0), "pic_enable_timing")
+    def enable_timing(self, on: bool = True, group: int = 1) -> None:
+        """Time kernel launches with HIP events: each launch, or with ``group``
+        > 1 runs of up to ``group`` back-to-back sweep launches per event pair
+        (every sweep is still timed; the pairs cost the stream less)."""
+        _lib.check(self.L.pic_enable_timing(self.handle, max(1, int(group)) if on else 0), "pic_enable_timing")
 
     def set_graphs(self, on: bool = True) -> None:
         """Replay run_steps' launches through graphs, or launch them one by one (the default)."""

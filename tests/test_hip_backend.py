@@ -142,3 +142,34 @@ def test_run_pulse_equals_upload_run_read():
     for a, b in zip(f_one, f_three):
         np.testing.assert_array_equal(a, b)
     assert one[:, 0].sum() > 0
+
+
+def test_grouped_timing_times_every_sweep_launch():
+    """Timing sweeps in runs of n launches per event pair: every sweep is
+    counted, deposit launches are timed one by one, the grouped mean is close
+    to the per-launch mean, and timing never changes a result."""
+    from configs import readme_config
+    from pulsed_ion_chamber_hip.pulses import draw_tracks
+    from pulsed_ion_chamber_hip.solver_hip import HipSimulation
+
+    cfg = readme_config(20.0, 0.0101, voltage_V=96.53, dose_rate_Gy_s=40.0)  # 18^2 x 106, 337 steps
+    n = cfg.total_time_steps
+    draw = draw_tracks(cfg)
+    out = {}
+    with HipSimulation(cfg, "fp32") as sim:
+        sim.upload_tracks(*draw)
+        for group in (0, 1, 5):
+            sim.reset()
+            sim.enable_timing(group > 0, group=max(group, 1))
+            sim.run_steps(0, n)
+            t = sim.timing()
+            out[group] = (t, sim.read_per_step(), *sim.read_fields())
+    assert out[0][0]["sweep_launches"] == 0
+    assert out[1][0]["sweep_launches"] == out[5][0]["sweep_launches"] == n
+    assert out[5][0]["deposit_launches"] == out[1][0]["deposit_launches"] > 0
+    mean1 = out[1][0]["sweep_ms"] / n
+    mean5 = out[5][0]["sweep_ms"] / n
+    assert 0.5 * mean1 < mean5 < 2.0 * mean1
+    for group in (1, 5):
+        for a, b in zip(out[0][1:], out[group][1:]):
+            np.testing.assert_array_equal(a, b)
