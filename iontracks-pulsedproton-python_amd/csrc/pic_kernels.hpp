@@ -277,7 +277,7 @@ constexpr int kBufferWord3 = 0x00020000;
 template <typename S, typename A, int TJ, int V, bool REFLECT>
 __device__ __forceinline__ void sweep_body(const SweepArgs &a, const int logical, const bool DEP) {
     static_assert(V == 1 || ((V == 2 || V == 4) && sizeof(S) == 4), "two or four values per lane are FP32 layouts");
-    constexpr int WIN = window_outputs(V);  // k step between windows
+    constexpr int WIN = window_outputs(V);  // k step between windows (V = 2: a.win_step, 124 or 126)
     const int kg = logical % a.n_kg;
     const int rest = logical / a.n_kg;
     const int jt = rest % a.n_jt;
