@@ -184,7 +184,8 @@ def test_one_fp32_sweep_stays_within_fp32_rounding_of_the_oracle(wall, grid_um, 
 
 
 @pytest.mark.parametrize("grid_um,gap_cm", FP32_GRIDS)
-def test_fp32_one_two_and_four_values_per_lane_give_the_same_fields(grid_um, gap_cm):
+@pytest.mark.parametrize("wall", ["absorbing", "reflecting"])
+def test_fp32_one_two_and_four_values_per_lane_give_the_same_fields(wall, grid_um, gap_cm):
     """The FP32 layouts (4-, 8- and 16-byte lanes in windows of 62, 124 and
     248 values over padded rows) run the same voxel update: a whole run must
     agree bit for bit, k end planes included. Small grids pad FP32 rows to an
@@ -193,7 +194,8 @@ def test_fp32_one_two_and_four_values_per_lane_give_the_same_fields(grid_um, gap
     are refused elsewhere."""
     from pulsed_ion_chamber_hip.pulses import draw_tracks
 
-    config = SimulationConfig(**{**CC, "grid_size_um": grid_um, "electrode_gap_cm": gap_cm}, **TWO_SPECIES)
+    config = SimulationConfig(**{**CC, "grid_size_um": grid_um, "electrode_gap_cm": gap_cm}, **TWO_SPECIES,
+                              lateral_boundary=wall)
     draw = draw_tracks(config)
     four = config.no_z_with_buffer % 4 in (0, 3)
     if not four:
@@ -209,5 +211,10 @@ def test_fp32_one_two_and_four_values_per_lane_give_the_same_fields(grid_um, gap
     for other in out[1:]:
         np.testing.assert_array_equal(out[0][1], other[1])
         np.testing.assert_array_equal(out[0][2], other[2])
-        # the scored sums add the same terms in another order
-        np.testing.assert_allclose(out[0][0], other[0], rtol=1e-12, atol=1e-300)
+        # the scored FP64 sums add the same terms in another order (four values
+        # per lane fold a lane's values first): rounding of up to ~3e5 terms,
+        # n * eps ~ 4e-11 of a sum, and once the gap has emptied the totals are
+        # residues of mixed sign far below their peak, so the bound is absolute
+        for col in range(out[0][0].shape[1]):
+            peak = np.abs(out[0][0][:, col]).max()
+            np.testing.assert_allclose(out[0][0][:, col], other[0][:, col], rtol=1e-10, atol=1e-12 * peak)
