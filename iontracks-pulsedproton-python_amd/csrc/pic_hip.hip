@@ -87,14 +87,17 @@ struct pic_sim {
     int prec = PIC_FP64;
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t gstream = nullptr;  // gathers of the next segment, beside this one's sweeps
+    hipEvent_t seg_go = nullptr;    // a segment's first sweep is done: the next gather may start
+    hipEvent_t seg_dens = nullptr;  // a segment's gather is done: its sweeps may start
     size_t elem = 8;
     int64_t pitch = 0;  // row pitch in elements (nzb; FP32 rounded up to even, or to a multiple of four)
     int64_t n_vox = 0;  // allocated elements per array, no_xy^2 * pitch
     void *cur_p = nullptr, *cur_n = nullptr, *nxt_p = nullptr, *nxt_n = nullptr;
-    double *dens = nullptr;        // [batch + 1 slots][no_xy^2]
+    double *dens = nullptr;        // [dens_slots][no_xy^2]
     double *sweep_part = nullptr;  // [2 slots][sweep_part_cap][3]
     int64_t sweep_part_cap = 0;
-    double *dep_part = nullptr;    // [batch + 1 slots][gather tiles]
+    double *dep_part = nullptr;    // [dens_slots][gather tiles]
     int64_t batch = 1;             // steps per gather launch
     int64_t *offsets_d = nullptr;
     // a step whose partials are written but not yet folded into its record
@@ -508,7 +511,7 @@ pic::BatchTable<D> batch_table(const D *descs, const int *offs, int n_cfg) {
 // With timing on, launches are bracketed by pairs of events on the handle's
 // stream: each launch, or runs of up to timing_group back-to-back sweep
 // launches; pic_get_timing sums the pairs per kind.
-void record(pic_sim *s, int kind) {
+void record(pic_sim *s, int kind, hipStream_t st = nullptr) {
     if (!s->timing || !s->in_run) return;
     if (s->ev_used == s->ev_pool.size()) {
         hipEvent_t e;
@@ -521,7 +524,7 @@ void record(pic_sim *s, int kind) {
     }
     s->ev_kind[s->ev_used] = kind;
     s->ev_count[s->ev_used] = 1;
-    if (hipEventRecord(s->ev_pool[s->ev_used], s->stream) == hipSuccess) ++s->ev_used;
+    if (hipEventRecord(s->ev_pool[s->ev_used], st ? st : s->stream) == hipSuccess) ++s->ev_used;
 }
 
 // end the open run of timed sweeps, if any
@@ -559,10 +562,10 @@ int emit(pic_sim *s, const KLaunch &l, hipStream_t st) {
         return PIC_OK;
     }
     if (s->timing && s->in_run) close_group(s);  // another kind of launch ends a run of sweeps
-    record(s, l.kind);
+    record(s, l.kind, st);
     const int rc = launch_now(l, st);
     if (rc) return rc;
-    record(s, l.kind);
+    record(s, l.kind, st);
     return PIC_OK;
 }
 
@@ -667,7 +670,13 @@ int64_t gather_blocks(const pic_sim *s) {
     return nt * nt;
 }
 
-// arguments of the gather of steps from `first` (slot = step mod (batch+1));
+// Density and deposition-partial slots: step t uses slot t % dens_slots. Two
+// gather batches and one step: the gather of the next segment runs while this
+// segment's sweeps read theirs, and this segment's first launch still folds
+// the previous segment's last step.
+int64_t dens_slots(int64_t batch) { return 2 * batch + 1; }
+
+// arguments of the gather of steps from `first` (slot = step mod dens_slots);
 // offsets are device offsets into xs/ys
 pic::GatherArgs make_gather_args(const pic_sim *s, const double *xs, const double *ys, const int64_t *offsets,
                                  int64_t first) {
@@ -676,7 +685,7 @@ pic::GatherArgs make_gather_args(const pic_sim *s, const double *xs, const doubl
     g.ys = ys;
     g.offsets = offsets;
     g.first_step = first;
-    g.n_slots = s->batch + 1;
+    g.n_slots = dens_slots(s->batch);
     g.dens = s->dens;
     g.partials = s->dep_part;
     g.no_xy = (int)s->p.no_xy;
@@ -947,7 +956,7 @@ int adjoint_recompute_sweep(pic_adjoint *ad, int64_t t, int64_t src) {
     const int64_t xy = s->p.no_xy * s->p.no_xy;
     const bool dep = s->offsets[t + 1] > s->offsets[t];
     const int n_tiles = (int)gather_blocks(s);
-    const int64_t slot = t % (s->batch + 1);
+    const int64_t slot = t % dens_slots(s->batch);
     s->pending = pic_sim::Pending{};
     pic::SweepArgs a;
     int rc = make_sweep_args(s, sh, 0, dep, s->dens + slot * xy, s->dep_part + slot * n_tiles, n_tiles, ad->scratch, a);
@@ -972,7 +981,7 @@ int adjoint_deriv_gather(pic_adjoint *ad, int64_t first, int64_t count) {
     g.ys = s->ys;
     g.offsets = s->offsets_d;
     g.first_step = first;
-    g.n_slots = s->batch + 1;
+    g.n_slots = dens_slots(s->batch);
     g.no_xy = (int)s->p.no_xy;
     g.no_z = (int)s->p.no_z;
     g.ntj = (int)ceil_div(s->p.no_xy, pic::kGT);
@@ -1003,7 +1012,7 @@ int adjoint_step(pic_adjoint *ad, int64_t t, int64_t src, int cur) {
     const double *in = ad->store + src * 2 * s->n_vox;
     a.sp = in;
     a.sn = in + s->n_vox;
-    a.dens = dep ? s->dens + (t % (s->batch + 1)) * xy : ad->zeros;
+    a.dens = dep ? s->dens + (t % dens_slots(s->batch)) * xy : ad->zeros;
     a.dgf = dep ? ad->dgf + (t % ad->K) * xy : ad->zeros;
     a.dh = dep ? ad->dh + (t % ad->K) * xy : ad->zeros;
     a.mp = ad->mu + (2 * cur + 0) * s->n_vox;
@@ -1072,7 +1081,7 @@ int pic_mem_required(const pic_params *p, int precision, size_t *bytes) {
     const size_t vox = (size_t)p->no_xy * p->no_xy * row_pitch(p, precision);
     const size_t xy = (size_t)p->no_xy * p->no_xy;
     const size_t gt = (size_t)ceil_div(p->no_xy, pic::kGT);
-    const size_t b = (size_t)batch_steps(p) + 1;
+    const size_t b = (size_t)dens_slots(batch_steps(p));
     *bytes = 4 * vox * elem + b * (xy * 8 + gt * gt * 8) + 2 * partial_cap_bytes(p) + 256;
     return PIC_OK;
 }
@@ -1104,8 +1113,12 @@ int pic_create(const pic_params *p, int precision, pic_sim **out) {
         pic_destroy(s);
         return code;
     };
-    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&s->gstream, hipStreamNonBlocking) != hipSuccess)
         return cleanup(fail(PIC_E_HIP, "hipStreamCreate failed"));
+    if (hipEventCreateWithFlags(&s->seg_go, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&s->seg_dens, hipEventDisableTiming) != hipSuccess)
+        return cleanup(fail(PIC_E_HIP, "hipEventCreate failed"));
     void **bufs[4] = {&s->cur_p, &s->cur_n, &s->nxt_p, &s->nxt_n};
     for (void **b : bufs) {
         if (hipMalloc(b, s->n_vox * s->elem) != hipSuccess) return cleanup(fail(PIC_E_NOMEM, "hipMalloc of a carrier array failed"));
@@ -1114,7 +1127,7 @@ int pic_create(const pic_params *p, int precision, pic_sim **out) {
     const size_t xy = (size_t)p->no_xy * p->no_xy;
     s->sweep_part_cap = (int64_t)(partial_cap_bytes(p) / (3 * sizeof(double)));
     s->batch = batch_steps(p);
-    const int64_t slots = s->batch + 1;
+    const int64_t slots = dens_slots(s->batch);
     if (hipMalloc(&s->dens, slots * xy * 8) != hipSuccess ||
         hipMalloc(&s->sweep_part, 2 * s->sweep_part_cap * 3 * 8) != hipSuccess ||
         hipMalloc(&s->dep_part, slots * gather_blocks(s) * 8) != hipSuccess ||
@@ -1131,11 +1144,15 @@ int pic_create(const pic_params *p, int precision, pic_sim **out) {
 int pic_destroy(pic_sim *s) {
     if (!s) return PIC_OK;
     if (s->stream) hipStreamSynchronize(s->stream);
+    if (s->gstream) hipStreamSynchronize(s->gstream);
     void *bufs[] = {s->cur_p, s->cur_n, s->nxt_p, s->nxt_n, s->dens, s->sweep_part, s->dep_part,
                     s->scratch4, s->per_step, s->xs, s->ys, s->offsets_d};
     for (void *b : bufs)
         if (b) hipFree(b);
     for (hipEvent_t e : s->ev_pool) hipEventDestroy(e);
+    if (s->seg_go) hipEventDestroy(s->seg_go);
+    if (s->seg_dens) hipEventDestroy(s->seg_dens);
+    if (s->gstream) hipStreamDestroy(s->gstream);
     if (s->stream) hipStreamDestroy(s->stream);
     delete s;
     return PIC_OK;
@@ -1150,6 +1167,7 @@ int pic_upload_tracks(pic_sim *s, const double *xs, const double *ys, int64_t n_
     for (int64_t i = 0; i < n_steps; ++i)
         if (offsets[i + 1] < offsets[i]) return fail(PIC_E_INVALID, "offsets must be non-decreasing");
     HIP_TRY(hipStreamSynchronize(s->stream));
+    HIP_TRY(hipStreamSynchronize(s->gstream));  // a side gather reads the track arrays
     s->pending = pic_sim::Pending{};
     if (s->xs) hipFree(s->xs);
     if (s->ys) hipFree(s->ys);
@@ -1182,29 +1200,61 @@ int pic_run_steps(pic_sim *s, int64_t first, int64_t count) {
         return fail(PIC_E_INVALID, "steps [%lld, %lld) outside the uploaded %lld", (long long)first,
                     (long long)(first + count), (long long)s->n_steps);
     HIP_TRY(hipStreamSynchronize(s->stream));  // events of an earlier call may still be pending
+    HIP_TRY(hipStreamSynchronize(s->gstream));
     s->ev_used = 0;
     s->group_open = 0;
     s->in_run = true;
     int rc = PIC_OK;
     const int n_tiles = (int)gather_blocks(s);
     const int64_t xy = s->p.no_xy * s->p.no_xy;
-    const int64_t slots = s->batch + 1;
-    // one segment per gather batch: its gather and sweeps are collected, then
-    // replayed through graph chains (launched one by one while timing)
+    const int64_t slots = dens_slots(s->batch);
+    const int64_t end = first + count;
+    // One segment per gather batch. Launched one by one, a segment's gather
+    // runs on the side stream beside the previous segment's sweeps: it starts
+    // once that segment's first sweep has folded the step before it, and the
+    // segment's sweeps wait for it. With graph replay a segment's gather and
+    // sweeps are collected, then replayed through graph chains in order.
     const bool chained = s->graphs && !s->timing;
-    for (int64_t b0 = first; b0 < first + count && rc == PIC_OK; b0 += s->batch) {
-        const int64_t b1 = std::min(first + count, b0 + s->batch);
+    // the gather of segment [g0, ...) on the side stream: 1 queued, 0 no tracks, else an error
+    auto side_gather = [&](int64_t g0) -> int {
+        const int64_t g1 = std::min(end, g0 + s->batch);
+        if (g0 >= end || s->offsets[g1] <= s->offsets[g0]) return 0;
+        int r = enqueue_gather(s, s->xs, s->ys, s->offsets_d, g0, g1 - g0, s->gstream);
+        if (!r && hipEventRecord(s->seg_dens, s->gstream) != hipSuccess) r = fail(PIC_E_HIP, "hipEventRecord failed");
+        return r ? r : 1;
+    };
+    bool gather_queued = false;
+    if (!chained) {
+        const int g = side_gather(first);
+        if (g == 1) gather_queued = true;
+        else if (g) rc = g;
+    }
+    for (int64_t b0 = first; b0 < end && rc == PIC_OK; b0 += s->batch) {
+        const int64_t b1 = std::min(end, b0 + s->batch);
         if (chained) {
             s->seg.clear();
             s->sink = &s->seg;
+            if (s->offsets[b1] > s->offsets[b0]) rc = enqueue_gather(s, s->xs, s->ys, s->offsets_d, b0, b1 - b0);
+        } else if (gather_queued) {
+            if (hipStreamWaitEvent(s->stream, s->seg_dens, 0) != hipSuccess) rc = fail(PIC_E_HIP, "hipStreamWaitEvent failed");
+            gather_queued = false;
         }
-        if (s->offsets[b1] > s->offsets[b0]) rc = enqueue_gather(s, s->xs, s->ys, s->offsets_d, b0, b1 - b0);
         for (int64_t step = b0; step < b1 && rc == PIC_OK; ++step) {
             const bool dep = s->offsets[step + 1] > s->offsets[step];
             const int64_t slot = step % slots;
             rc = enqueue_sweep(s, step, dep, s->dens + slot * xy, s->dep_part + slot * n_tiles, n_tiles,
                                s->per_step + 4 * step);
             if (!rc) swap_buffers(s);
+            if (!rc && !chained && step == b0 && b1 < end) {
+                if (hipEventRecord(s->seg_go, s->stream) != hipSuccess ||
+                    hipStreamWaitEvent(s->gstream, s->seg_go, 0) != hipSuccess) {
+                    rc = fail(PIC_E_HIP, "ordering the side stream failed");
+                } else {
+                    const int g = side_gather(b1);
+                    if (g == 1) gather_queued = true;
+                    else if (g) rc = g;
+                }
+            }
         }
         s->sink = nullptr;
         if (chained && !rc) rc = submit(s->seg, s->device, s->stream);
@@ -1609,7 +1659,7 @@ int pic_batch_run(pic_batch *b, int64_t first, int64_t count) {
                 const SweepShape &sh = b->shapes[ci][ord[q]];
                 const int n_tiles = (int)gather_blocks(s);
                 const int64_t xy = s->p.no_xy * s->p.no_xy;
-                const int64_t slot = t % (s->batch + 1);
+                const int64_t slot = t % dens_slots(s->batch);
                 const bool dep = s->offsets[t + 1] > s->offsets[t];
                 rc = make_sweep_args(s, sh, t, dep, s->dens + slot * xy, s->dep_part + slot * n_tiles, n_tiles,
                                      s->per_step + 4 * t, descs[(t - s0) * n + q]);
