@@ -1406,6 +1406,7 @@ int pic_write_fields(pic_sim *s, const double *pos, const double *neg) {
 int pic_synchronize(pic_sim *s) {
     if (!s) return fail(PIC_E_INVALID, "sim is NULL");
     HIP_TRY(hipStreamSynchronize(s->stream));
+    HIP_TRY(hipStreamSynchronize(s->gstream));  // the side stream's gathers (a run waits for them itself)
     return PIC_OK;
 }
 
