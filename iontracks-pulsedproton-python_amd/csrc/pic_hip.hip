@@ -1608,7 +1608,10 @@ int pic_batch_run(pic_batch *b, int64_t first, int64_t count) {
     if (first < 0 || count < 0 || first + count > longest)
         return fail(PIC_E_INVALID, "steps [%lld, %lld) outside the longest uploaded run (%lld)", (long long)first,
                     (long long)(first + count), (long long)longest);
-    if (!b->is_lane && !b->timing) {
+    // lanes drive launches from two host threads: not while timing, and not
+    // with graph replay -- graph chains driven from two threads at once have
+    // had a queried event refused by the runtime ("stream is capturing")
+    if (!b->is_lane && !b->timing && !b->graphs) {
         std::vector<int> part;
         if (lane_split(b, part)) return run_lanes(b, part, first, count);
     }

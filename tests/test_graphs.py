@@ -65,6 +65,10 @@ def test_graph_replay_after_another_grid():
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_batch_graph_replay_matches_direct_launches(precision):
+    """A batch with graph replay runs as one lane (graph chains are driven
+    from one host thread), the direct run of four configurations as two: the
+    fields are the same bit for bit, the per-step sums are folded over another
+    partition of blocks and agree to rounding."""
     cfgs = [
         readme_config(20.0, 0.0146, voltage_V=50.0, dose_rate_Gy_s=2.0),
         readme_config(16.0, 0.0077, voltage_V=69.47, dose_rate_Gy_s=2.0),
@@ -87,5 +91,8 @@ def test_batch_graph_replay_matches_direct_launches(precision):
             for s in sims:
                 s.close()
     for a, b in zip(out[False], out[True]):
-        for x, y in zip(a, b):
-            np.testing.assert_array_equal(x, y)
+        np.testing.assert_array_equal(a[1], b[1])
+        np.testing.assert_array_equal(a[2], b[2])
+        for col in range(a[0].shape[1]):
+            peak = np.abs(a[0][:, col]).max()
+            np.testing.assert_allclose(b[0][:, col], a[0][:, col], rtol=1e-13, atol=1e-14 * peak)
