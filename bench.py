@@ -6,32 +6,39 @@ sweep of both species with its scored reduction, the lateral-ring update and
 the per-step reduction. The timed region starts with every track of the run
 already resident in HBM and ends when the last step's record is written.
 
-Default workload (the mid-size single solve): 56.2 MeV protons, 300 V over a
-0.2 cm gap, 540 us pulses at 50 Hz, 8.91 Gy/s, library-default physics, 5 um
-voxels over a sampled radius of 0.05 cm (208 x 208 x 406 voxels, 3388 steps),
-seed 1, FP32 fields with FP64 recombination and scored sums. The same run in
-FP64 fields is timed too, and the FP32 k_s error against it is reported.
-``--workload c4`` runs the 5 um full electrode (1066 x 1066 x 410,
-4580 steps, the reference's published A100 headline); ``--workload c5`` the
-2 um full electrode (2656 x 2656 x 1010 = 7.1 G voxels, 114 GB of FP32
-fields held in HBM; give it --steps, the whole run is 13 019 steps).
+Default workload (the headline): the 5 um full electrode of the Classic
+Markus chamber -- AIC-144 physics, 56.2 MeV/u protons, 300 V over 0.2 cm,
+540 us pulses at 50 Hz, 10 Gy/s to water, r 0.265 cm, 1066 x 1066 x 410 =
+466 M voxels, 4580 steps, 24.6 M tracks, seed 20260527 -- with FP32 fields
+and FP64 recombination and scored sums. ``--steps K`` times K steps from the
+start of the run; outside the timed region the whole run is then done in
+FP32 and in FP64 to report k_s, the FP32 k_s error and the FP64 k_s against
+the reference's published value (1.1104725894530176 on its GPU backend).
 
-The saturation curve (32 configurations: 8 voltages 50-500 V x 4 dose rates,
-grids 16^2..47^2 x 106..206, 243..2505 steps) is timed as one batch against 32
-sequential plain runs of the same configurations, and its FP32 k_s values are
-compared with an FP64 batch and with the CPU oracle; the default line carries
-it as ``curve``, ``--workload c2`` makes it the headline.
+Other workloads: ``c1`` the mid-size solve (library-default physics, 5 um
+over r 0.05 cm, 208 x 208 x 406, 3388 steps, seed 1; also reported as the
+``mid_size`` key of the default line); ``c0`` the README example (24 x 24 x
+206); ``c5`` the 2 um full electrode (2656 x 2656 x 1010 = 7.1 G voxels,
+114 GB of FP32 fields; give it --steps, the whole run is 13 019 steps);
+``c2`` the 32-point saturation curve (8 voltages 50-500 V x 4 dose rates,
+grids 16^2..47^2 x 106..206, 243..2505 steps) timed as one batch against 32
+sequential plain runs of the same configurations (the ``curve`` key of the
+default line); ``c3`` the adjoint (dk_s/dtheta over 3 pulses of a 48^2 x 206
+grid, FP64) with its Richardson-extrapolated finite-difference check.
 
 Prints one JSON line. ``value`` is interior voxel-steps per second over the
 timed region; ``roofline`` divides the sweep kernel's compulsory HBM bytes
 (read n+ and n-, write n+' and n-': 16 B per interior voxel in FP32, 32 B in
-FP64) by its average duration, measured with HIP events over the timed
-region: one event pair around each run of 16 back-to-back sweep launches, so
-every launch is timed (with the ~1 us between launches) while the pairs cost
-the run under 1 % (one pair per launch would cost ~9 % of a mid-size step);
-``cpu_baseline`` times the CPU oracle (a C/OpenMP
-restatement of the reference's batched backend) on a bounded prefix of the
-same run.
+FP64) by its average duration, measured with HIP events on the handle's
+stream over the timed region: one event pair around each run of 16
+back-to-back sweep launches, so every launch is timed (with the ~1 us
+between launches) while the pairs cost the run under 1 %; ``traffic`` is
+the copy-calibrated rocprofv3 FETCH_SIZE + WRITE_SIZE of the same kernel from
+the committed profile. ``cpu_baseline`` times the CPU oracle (a C/OpenMP
+restatement of the reference's batched backend, compiled here with
+-march=native) on a bounded prefix of the same run at 1, 2, 4, ... threads up
+to the cores this process may use, and reports the best, with the host's CPU
+model, NUMA node count and L3 size.
 """
 
 import argparse
@@ -109,26 +116,75 @@ def traffic_from_profiles(kernel_tag):
         return None
 
 
+def host_description():
+    """The CPU the baseline runs on: logical CPUs, the ones this process may
+    use, the thread cap the environment sets, and lscpu's model, sockets,
+    NUMA nodes and L3."""
+    import subprocess
+
+    usable = len(os.sched_getaffinity(0))
+    cap = os.environ.get("OMP_NUM_THREADS", "")
+    threads = min(usable, int(cap)) if cap.isdigit() and int(cap) > 0 else usable
+    info = {"nproc": os.cpu_count(), "usable_cpus": usable, "omp_num_threads_env": cap or None,
+            "threads_available": threads}
+    try:
+        text = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+    except (OSError, subprocess.SubprocessError):
+        text = ""
+    keys = {"Model name": "model", "Socket(s)": "sockets", "Core(s) per socket": "cores_per_socket",
+            "Thread(s) per core": "threads_per_core", "NUMA node(s)": "numa_nodes", "L3 cache": "l3"}
+    for line in text.splitlines():
+        name, _, value = line.partition(":")
+        if name.strip() in keys and keys[name.strip()] not in info:
+            info[keys[name.strip()]] = value.strip()
+    return info
+
+
 def cpu_baseline(cfg, schedule, xs, ys, budget_s):
+    """The CPU oracle on the first steps of the same run, at 1, 2, 4, ...
+    threads up to every core this process may use (the reference shows that
+    fewer threads can win on some grids, docs/PERFORMANCE.md:180-202); the
+    best rate is the baseline. Each count runs a step budget sized from a
+    pilot at full width, so the leg takes about ``budget_s`` of loop time;
+    the loop is timed inside the oracle, without its allocation and first
+    touch. The oracle is compiled for this host (-march=native) when gcc
+    can, else the in-tree x86-64-v3 build is used."""
+    import subprocess
+    import tempfile
+
     from oracle import batched
 
-    threads = min(16, len(os.sched_getaffinity(0)))
+    host = host_description()
+    try:
+        batched.use_library(batched.build_native(Path(tempfile.mkdtemp(prefix="pic_oracle_native_"))))
+        build = "-O3 -march=native"
+    except (OSError, subprocess.SubprocessError):
+        build = "-O3 -march=x86-64-v3 (in-tree build; native compile failed)"
     nvox = interior_voxels(cfg)
-    probe = min(20, cfg.total_time_steps)
-    t0 = time.perf_counter()
-    batched.run_per_step(cfg, schedule, xs, ys, n_steps=probe, nthreads=threads, fields=False)
-    rate = probe / (time.perf_counter() - t0)
-    sample = int(min(cfg.total_time_steps, max(probe, rate * budget_s)))
-    t0 = time.perf_counter()
-    batched.run_per_step(cfg, schedule, xs, ys, n_steps=sample, nthreads=threads, fields=False)
-    elapsed = time.perf_counter() - t0
+    total = cfg.total_time_steps
+    top = host["threads_available"]
+    counts = sorted({1 << k for k in range(top.bit_length()) if (1 << k) <= top} | {top})
+    pilot = min(2, total)
+    batched.run_per_step(cfg, schedule, xs, ys, n_steps=pilot, nthreads=top, fields=False)
+    rate_top = pilot / max(batched.last_loop_seconds(), 1e-9)
+    share = budget_s / len(counts)
+    sweep = []
+    for t in counts:
+        n = int(min(total, max(1, share * rate_top * t / top)))
+        batched.run_per_step(cfg, schedule, xs, ys, n_steps=n, nthreads=t, fields=False)
+        sec = batched.last_loop_seconds()
+        sweep.append({"threads": t, "steps": n, "loop_s": sec, "voxel_steps_per_s": nvox * n / sec})
+    best = max(sweep, key=lambda r: r["voxel_steps_per_s"])
     return {
-        "value": nvox * sample / elapsed,
+        "value": best["voxel_steps_per_s"],
         "unit": "voxel-steps/s",
-        "cores": threads,
+        "cores": best["threads"],
         "kind": "port",
-        "sample": f"first {sample} of {cfg.total_time_steps} steps of the same run (deposition + sweep + ring), "
-                  f"FP64 fields, {threads} OpenMP threads, {elapsed:.1f} s",
+        "sample": f"first {best['steps']} of {total} steps of the same run (deposition + sweep + ring), FP64 fields, "
+                  f"{best['threads']} OpenMP threads (best of {counts}), {best['loop_s']:.1f} s of loop",
+        "oracle_build": build,
+        "threads_sweep": sweep,
+        "host": host,
     }
 
 
@@ -230,7 +286,7 @@ def curve_bench(precision, cpu_budget_s):
     if cpu_budget_s > 0:
         from oracle import batched as oracle_batched
 
-        threads = min(16, len(os.sched_getaffinity(0)))
+        threads = host_description()["threads_available"]
         t0 = time.perf_counter()
         done = 0
         dks = rel64 = 0.0
@@ -257,8 +313,9 @@ def curve_bench(precision, cpu_budget_s):
 def adjoint_bench(fd: bool = True, pulses: int = 3):
     """dk_s/dtheta on the 10 um, r 0.02 cm grid over 3 pulses (FP64 fields,
     checkpoints every 64 steps): forward and reverse time, device memory, and
-    the gradient against FP64 central finite differences (relative step 1e-6,
-    dt and step count frozen) on all six parameters."""
+    the gradient against Richardson-extrapolated FP64 central differences
+    (relative steps 1e-3 and 2e-3, dt and step count frozen) on all six
+    parameters."""
     from configs import readme_config
     from pulsed_ion_chamber_hip.adjoint import PARAMS, KsModel
 
@@ -291,11 +348,120 @@ def adjoint_bench(fd: bool = True, pulses: int = 3):
     out["memory_over_plain"] = out["adjoint_device_bytes"] / out["plain_state_bytes"]
     if fd:
         t0 = time.perf_counter()
-        fdv = model.finite_difference(rel=1e-6)
+        fdv = model.finite_difference_richardson(rel=1e-3)
         out["fd_s"] = time.perf_counter() - t0
+        out["fd_method"] = "Richardson-extrapolated central differences, (4 D(h) - D(2h)) / 3, h = 1e-3 theta"
         out["fd_dks_dtheta"] = dict(zip(PARAMS, fdv.tolist()))
-        out["max_rel_err_vs_fd"] = float(np.max(np.abs(grad - fdv) / np.abs(fdv)))
+        rel = np.abs(grad - fdv) / np.abs(fdv)
+        out["rel_err_vs_fd"] = dict(zip(PARAMS, rel.tolist()))
+        out["max_rel_err_vs_fd"] = float(np.max(rel))
     return cfg, out
+
+
+
+def gather_batch_steps(cfg):
+    """Steps per gather launch, as the library sizes it: as many as fit in
+    256 MB of FP64 density, at most 128."""
+    return max(1, min(128, (256 << 20) // (cfg.no_xy * cfg.no_xy * 8)))
+
+
+def single_run_line(cfg, workload_key, schedule, xs, ys, precision, steps, warmup):
+    """Time ``steps`` steps from the start of the run (after ``warmup``
+    untimed steps and a reset) with every track resident in HBM; the sweep's
+    roofline from its HIP-event timing over the same region."""
+    nvox = interior_voxels(cfg)
+    elapsed, timing, ks = gpu_run(cfg, schedule, xs, ys, precision, steps, warmup)
+    bytes_per_voxel = 16 if precision == "fp32" else 32
+    sweep_avg_ms = timing["sweep_ms"] / max(1, timing["sweep_launches"])
+    achieved = nvox * bytes_per_voxel / (sweep_avg_ms * 1e-3) / 1e9
+    return {
+        "value": nvox * steps / elapsed,
+        "ms_per_step": elapsed / steps * 1e3,
+        "wall_s": elapsed,
+        "ks": ks,
+        "timing": timing,
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / PEAK_GBS,
+            "traffic": traffic_from_profiles(f"sweep_{workload_key}_{precision}"),
+            "kernel": "sweep_kernel",
+            "bytes_per_interior_voxel": bytes_per_voxel,
+            "avg_launch_ms": sweep_avg_ms,
+            "launches": timing["sweep_launches"],
+            "launches_per_event_pair": TIMING_GROUP,
+        },
+        "breakdown_ms_per_step": {
+            # a segment's gather (one launch per batch of steps) runs on a
+            # second stream beside the previous segment's sweeps; with a
+            # single segment there is nothing beside which to run
+            "deposit_overlaps_sweep": steps > gather_batch_steps(cfg),
+            "sweep": sweep_avg_ms,
+            "deposit": timing["deposit_ms"] / steps,
+            "deposit_launches": timing["deposit_launches"],
+            "ring_and_reduce": timing["ring_reduce_ms"] / steps,
+        },
+    }
+
+
+def published_ks(workload_key):
+    """The reference's published full-precision k_s for a workload, from the
+    committed fixture of its data files (profiling/data/gpu_athena/*.json)."""
+    where = {"c4": (0.265, 5.0)}.get(workload_key)
+    if where is None:
+        return None
+    table = json.loads((ROOT / "tests" / "golden" / "published.json").read_text())
+    for row in table["aic_runs"]:
+        if (row["sampled_radius_cm"], row["grid_size_um"]) == where:
+            return {"gpu_ks": row["gpu_ks"], "cpu_ks": row.get("cpu_ks"),
+                    "source": "profiling/data/gpu_athena/gpu_5um_full_electrode.json (A100, FP64, 156.3 s)"}
+    return None
+
+
+def whole_runs(cfg, workload_key, schedule, xs, ys, precision, timed_line, with_fp64):
+    """The whole run in the benchmark precision (reusing the timed one when
+    it was whole) and, for FP32, in FP64: k_s of each, the FP32 error and the
+    FP64 value against the reference's published one. Untimed for ``value``;
+    each run's wall time is reported."""
+    total = cfg.total_time_steps
+    nvox = interior_voxels(cfg)
+    run = timed_line or single_run_line(cfg, workload_key, schedule, xs, ys, precision, total, 0)
+    out = {"ks": run["ks"], "full_run": {
+        "precision": precision, "steps": total, "wall_s": run["wall_s"], "ms_per_step": run["ms_per_step"],
+        "sweep_avg_ms": run["roofline"]["avg_launch_ms"], "sweep_frac_of_peak": run["roofline"]["frac"],
+        "ks": run["ks"]}}
+    ks64 = run["ks"] if precision == "fp64" else None
+    if with_fp64 and precision == "fp32":
+        e64, t64, ks64 = gpu_run(cfg, schedule, xs, ys, "fp64", total, 0)
+        sw64 = t64["sweep_ms"] / max(1, t64["sweep_launches"])
+        out["fp64"] = {"value": nvox * total / e64, "wall_s": e64, "ms_per_step": e64 / total * 1e3, "ks": ks64,
+                       "sweep_avg_ms": sw64, "sweep_GBs": nvox * 32 / (sw64 * 1e-3) / 1e9}
+        out["dks_fp32_vs_fp64"] = abs(run["ks"] - ks64)
+    pub = published_ks(workload_key)
+    if pub is not None:
+        pub = dict(pub)
+        if ks64 is not None:
+            pub["rel_err_fp64_vs_gpu_ks"] = abs(ks64 - pub["gpu_ks"]) / pub["gpu_ks"]
+        if precision == "fp32":
+            pub["abs_err_fp32_vs_gpu_ks"] = abs(run["ks"] - pub["gpu_ks"])
+        out["ks_published"] = pub
+    return out
+
+
+def mid_size_line():
+    """The mid-size solve (5 um, r 0.05 cm, 208^2 x 406, 3388 steps), whole
+    run in FP32 (timed) and FP64 (for the k_s error)."""
+    from pulsed_ion_chamber_hip.pulses import draw_tracks
+
+    cfg, name = workload("c1")
+    schedule, xs, ys = draw_tracks(cfg)
+    line = single_run_line(cfg, "c1", schedule, xs, ys, "fp32", cfg.total_time_steps, 50)
+    e64, t64, ks64 = gpu_run(cfg, schedule, xs, ys, "fp64", cfg.total_time_steps, 0)
+    return {"workload": name, "grid": [cfg.no_xy, cfg.no_xy, cfg.no_z_with_buffer],
+            "steps": cfg.total_time_steps, "value": line["value"], "ms_per_step": line["ms_per_step"],
+            "roofline": line["roofline"], "ks": line["ks"], "ks_fp64": ks64, "dks_fp32_vs_fp64": abs(line["ks"] - ks64)}
 
 
 def main():
@@ -303,12 +469,14 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=0, help="timed steps (default: the whole run)")
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--workload", default="c1", choices=["c0", "c1", "c2", "c3", "c4", "c5"])
+    ap.add_argument("--workload", default="c4", choices=["c0", "c1", "c2", "c3", "c4", "c5"])
     ap.add_argument("--no-curve", action="store_true", help="skip the saturation-curve comparison")
     ap.add_argument("--adjoint-pulses", type=int, default=3, help="pulses of the adjoint workload (c3)")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
     ap.add_argument("--no-reference-precision", action="store_true", help="skip the FP64 comparison run")
-    ap.add_argument("--cpu-budget-s", type=float, default=12.0)
+    ap.add_argument("--no-full-run", action="store_true", help="skip the whole runs that give k_s")
+    ap.add_argument("--no-mid-size", action="store_true", help="skip the mid-size solve key of the c4 line")
+    ap.add_argument("--cpu-budget-s", type=float, default=25.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
 
@@ -386,34 +554,30 @@ def main():
         print(json.dumps(out))
         return
     cfg, wl_name = workload(args.workload)
-    if args.workload == "c5":
-        # the FP64 comparison run would need 228 GB of device memory for a
-        # partial run, the CPU oracle 228 GB of host memory: neither is run
-        args.no_cpu = True
-        args.no_reference_precision = True
     total = cfg.total_time_steps
     steps = total if args.steps <= 0 else min(args.steps, total)
+    if args.workload == "c5":
+        # the FP64 comparison run would need 228 GB of device memory, the CPU
+        # oracle 228 GB of host memory, and the whole FP32 run ~6 minutes:
+        # only the timed steps run
+        args.no_cpu = True
+        args.no_reference_precision = True
+        args.no_full_run = steps < total
     t_host = time.perf_counter()
     schedule, xs, ys = draw_tracks(cfg)
     host_draw_s = time.perf_counter() - t_host
-    nvox = interior_voxels(cfg)
     print(f"bench: {wl_name} {cfg.no_xy}x{cfg.no_xy}x{cfg.no_z_with_buffer}, {steps}/{total} steps, "
           f"{len(xs)} tracks, {args.precision}, device {device}", file=sys.stderr, flush=True)
 
-    elapsed, timing, ks = gpu_run(cfg, schedule, xs, ys, args.precision, steps, args.warmup)
-    bytes_per_voxel = 16 if args.precision == "fp32" else 32
-    sweep_avg_ms = timing["sweep_ms"] / max(1, timing["sweep_launches"])
-    achieved = nvox * bytes_per_voxel / (sweep_avg_ms * 1e-3) / 1e9
-    tag = f"sweep_{args.workload}_{args.precision}"
-    traffic = traffic_from_profiles(tag)
+    line = single_run_line(cfg, args.workload, schedule, xs, ys, args.precision, steps, args.warmup)
     out = {
         "metric": METRIC,
-        "value": nvox * steps / elapsed,
+        "value": line["value"],
         "unit": "voxel-steps/s",
         "n_gpus": 1,
         "steps": steps,
         "warmup": args.warmup,
-        "ms_per_step": elapsed / steps * 1e3,
+        "ms_per_step": line["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -422,55 +586,32 @@ def main():
         "config": {
             "workload": wl_name,
             "grid": [cfg.no_xy, cfg.no_xy, cfg.no_z_with_buffer],
-            "interior_voxels": nvox,
+            "interior_voxels": interior_voxels(cfg),
             "total_time_steps": total,
             "tracks": int(len(xs)),
             "fields": args.precision,
             "accumulators": "f64 (recombination product, deposition, scored sums)",
             "seed": cfg.seed,
         },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / PEAK_GBS,
-            "traffic": traffic,
-            "kernel": "sweep_kernel",
-            "bytes_per_interior_voxel": bytes_per_voxel,
-            "avg_launch_ms": sweep_avg_ms,
-            "launches": timing["sweep_launches"],
-            "launches_per_event_pair": TIMING_GROUP,
-        },
-        "breakdown_ms_per_step": {
-            # each segment's gather runs on a second stream beside the previous
-            # segment's sweeps, so "deposit" overlaps "sweep" instead of adding to it
-            "deposit_overlaps_sweep": True,
-            "sweep": sweep_avg_ms,
-            "deposit": timing["deposit_ms"] / steps,
-            "ring_and_reduce": timing["ring_reduce_ms"] / steps,
-        },
-        "ks": ks,
+        "roofline": line["roofline"],
+        "breakdown_ms_per_step": line["breakdown_ms_per_step"],
+        "ks": line["ks"],
         "host_track_draw_s": host_draw_s,
     }
-    if not args.no_reference_precision and args.precision == "fp32":
-        e64, t64, ks64 = gpu_run(cfg, schedule, xs, ys, "fp64", steps, 0)
-        sw64 = t64["sweep_ms"] / max(1, t64["sweep_launches"])
-        out["fp64"] = {
-            "value": nvox * steps / e64,
-            "ms_per_step": e64 / steps * 1e3,
-            "ks": ks64,
-            "sweep_avg_ms": sw64,
-            "sweep_GBs": nvox * 32 / (sw64 * 1e-3) / 1e9,
-        }
-        out["dks_fp32_vs_fp64"] = abs(ks - ks64) if steps == total else None
+    if not args.no_full_run:
+        print("bench: whole runs for k_s", file=sys.stderr, flush=True)
+        out.update(whole_runs(cfg, args.workload, schedule, xs, ys, args.precision,
+                              line if steps == total else None, not args.no_reference_precision))
     if not args.no_cpu:
+        print("bench: CPU baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(cfg, schedule, xs, ys, args.cpu_budget_s)
-    if not args.no_curve and args.workload == "c1":
+    if args.workload == "c4" and not args.no_mid_size:
+        print("bench: mid-size solve", file=sys.stderr, flush=True)
+        out["mid_size"] = mid_size_line()
+    if not args.no_curve and args.workload in ("c1", "c4"):
         print("bench: saturation curve", file=sys.stderr, flush=True)
         out["curve"] = curve_bench("fp32", 0.0 if args.no_cpu else args.cpu_budget_s)
     print(json.dumps(out))
-
 
 if __name__ == "__main__":
     main()

@@ -1177,6 +1177,14 @@ int pic_upload_tracks(pic_sim *s, const double *xs, const double *ys, int64_t n_
     s->offsets_d = nullptr;
     s->n_steps = 0;
     const size_t tb = (size_t)std::max<int64_t>(n_tracks, 1) * 8;
+    // checked before allocating, at 0.9 of free memory like the grid itself
+    // (the full electrode's 24.6 M tracks take 394 MB)
+    const size_t need = 2 * tb + (size_t)std::max<int64_t>(n_steps, 1) * 4 * 8 + (size_t)(n_steps + 1) * 8;
+    size_t fr = 0, tot = 0;
+    HIP_TRY(hipMemGetInfo(&fr, &tot));
+    if ((double)need > 0.9 * (double)fr)
+        return fail(PIC_E_NOMEM, "%lld tracks over %lld steps need %.3f GiB of device memory, %.3f GiB free",
+                    (long long)n_tracks, (long long)n_steps, need / 1073741824.0, fr / 1073741824.0);
     if (hipMalloc(&s->xs, tb) != hipSuccess || hipMalloc(&s->ys, tb) != hipSuccess ||
         hipMalloc(&s->per_step, (size_t)std::max<int64_t>(n_steps, 1) * 4 * 8) != hipSuccess ||
         hipMalloc(&s->offsets_d, (size_t)(n_steps + 1) * 8) != hipSuccess)
@@ -1262,6 +1270,14 @@ int pic_run_steps(pic_sim *s, int64_t first, int64_t count) {
     if (!rc) rc = flush_pending(s);
     close_group(s);
     s->in_run = false;
+    // Whatever happened above, the handle's own stream ends behind every
+    // gather queued on the side stream, so any later call that synchronises
+    // only that stream also waits for them (a gather still writing density
+    // slots the next call reuses would otherwise race with it).
+    if (hipEventRecord(s->seg_dens, s->gstream) != hipSuccess || hipStreamWaitEvent(s->stream, s->seg_dens, 0) != hipSuccess) {
+        if (!rc) rc = fail(PIC_E_HIP, "ordering the handle's stream after its side stream failed");
+    }
+    if (rc) s->pending = pic_sim::Pending{};  // a failed run leaves no step to fold
     return rc;
 }
 

@@ -15,7 +15,8 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent.parent
 SOURCES = [PKG.parent / "csrc" / "pic_hip.hip"]
-DEPENDS = SOURCES + [PKG.parent / "csrc" / "pic_kernels.hpp", ROOT / "include" / "pic_hip.h"]
+DEPENDS = SOURCES + [PKG.parent / "csrc" / "pic_kernels.hpp", PKG.parent / "csrc" / "pic_adjoint.hpp",
+                     ROOT / "include" / "pic_hip.h"]
 OUTPUT = PKG / "libpic_hip.so"
 ARCH = os.environ.get("PIC_HIP_ARCH", "gfx950")
 
@@ -43,7 +44,9 @@ PROBE_OUTPUT = PKG / "hbm_copy_probe"
 
 
 def build_probe(verbose: bool = False) -> Path:
-    """The copy-calibration executable used when profiling HBM traffic."""
+    """The copy-calibration executable behind the HBM traffic figures (a
+    measuring instrument, not part of the library): built only on request,
+    ``python -m pulsed_ion_chamber_hip._build probe``."""
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-O3", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
            "-o", str(PROBE_OUTPUT), str(PROBE_SOURCE)]
     if verbose:
@@ -70,6 +73,8 @@ def build(verbose: bool = False) -> Path:
 
 
 if __name__ == "__main__":
-    print(build(verbose=True))
-    print(build_probe(verbose=True))
+    if sys.argv[1:] == ["probe"]:
+        print(build_probe(verbose=True))
+    else:
+        print(build(verbose=True))
     sys.exit(0)

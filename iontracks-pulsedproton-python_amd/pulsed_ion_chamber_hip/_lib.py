@@ -10,11 +10,7 @@ from pathlib import Path
 
 import numpy as np
 
-import os
-
-# PIC_HIP_LIBRARY points at an alternative build of the same C ABI (A/B timing
-# of kernel variants); the in-tree library is the default.
-LIB_PATH = Path(os.environ.get("PIC_HIP_LIBRARY", Path(__file__).resolve().parent / "libpic_hip.so"))
+LIB_PATH = Path(__file__).resolve().parent / "libpic_hip.so"
 
 PIC_OK = 0
 PIC_E_INVALID = -1
@@ -194,6 +190,21 @@ def device_info():
         )
     check(rc, "pic_device_info")
     return name.value.decode(), free.value, total.value
+
+
+_LIB_DIGEST = None
+
+
+def library_digest() -> str:
+    """First 16 hex digits of the SHA-256 of the built library: identifies the
+    kernel set (a rebuilt library may accept other tilings, or run them at
+    other speeds)."""
+    global _LIB_DIGEST
+    if _LIB_DIGEST is None:
+        import hashlib
+
+        _LIB_DIGEST = hashlib.sha256(LIB_PATH.read_bytes()).hexdigest()[:16]
+    return _LIB_DIGEST
 
 
 def dp(a: np.ndarray):

@@ -250,7 +250,8 @@ def run_simulation_hip(
     _lib.open_library()
     _lib.device_info()
 
-    if tiling == "auto":
+    auto = tiling == "auto"
+    if auto:
         from .tuning import autotune
 
         tiling = tuple(autotune(config, precision)["tiling"])
@@ -261,7 +262,15 @@ def run_simulation_hip(
     last = total if max_steps is None else min(total, int(max_steps))
     with HipSimulation(config, precision) as sim:
         if tiling is not None:
-            sim.set_tiling(*tiling)
+            try:
+                sim.set_tiling(*tiling)
+            except ValueError:
+                if not auto:
+                    raise
+                # a cached tiling this library no longer accepts: measure again
+                from .tuning import autotune
+
+                sim.set_tiling(*autotune(config, precision, refresh=True)["tiling"])
         sim.upload_tracks(schedule, xs, ys)
         report_every = max(1, total // 20)
         loop_t0 = perf_counter()

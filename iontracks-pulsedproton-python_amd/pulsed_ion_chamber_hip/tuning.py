@@ -6,8 +6,8 @@ The library picks a tiling by a rule fitted to measurements on MI355X
 far from the measured ones may prefer another shape, so this module measures:
 it runs the first in-pulse steps of the configuration's own schedule under
 each candidate tiling (HIP events around every sweep launch), keeps the
-fastest, and caches it in a JSON file keyed by device, grid, precision and
-wall. Every tiling gives the same field values -- each voxel is updated by
+fastest, and caches it in a JSON file keyed by device, library build, grid,
+precision and wall (a rebuilt library may offer other tilings). Every tiling gives the same field values -- each voxel is updated by
 the same arithmetic -- so tuning changes speed only (and, within rounding,
 the order of the scored partial sums).
 
@@ -34,7 +34,8 @@ def cache_path() -> Path:
 
 def cache_key(config, precision: str) -> str:
     device = _lib.device_info()[0]
-    return f"{device}|{config.no_xy}x{config.no_xy}x{config.no_z_with_buffer}|{precision}|{config.lateral_boundary}"
+    grid = f"{config.no_xy}x{config.no_xy}x{config.no_z_with_buffer}"
+    return f"{device}|lib {_lib.library_digest()}|{grid}|{precision}|{config.lateral_boundary}"
 
 
 def _load(path: Path) -> dict:
@@ -63,14 +64,15 @@ def cached_tiling(config, precision: str) -> Optional[Tiling]:
 
 
 def autotune(config, precision: str = "fp32", steps: int = 48, repeats: int = 2,
-             tilings: Optional[Iterable[Tiling]] = None, use_cache: bool = True) -> dict:
+             tilings: Optional[Iterable[Tiling]] = None, use_cache: bool = True, refresh: bool = False) -> dict:
     """Measure the candidate tilings and return
     ``{"tiling": (rows, planes, values), "sweep_us": ..., "default_us": ..., "measured": {...}}``.
     With ``use_cache`` a cached result is returned without measuring, and a
-    new result is stored."""
+    new result is stored; ``refresh`` measures even when an entry exists and
+    replaces it."""
     key = cache_key(config, precision)
     path = cache_path()
-    if use_cache:
+    if use_cache and not refresh:
         entry = _load(path).get(key)
         if entry:
             return dict(entry, cached=True)

@@ -51,6 +51,26 @@ def build() -> None:
     subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
 
 
+def build_native(dest: Path) -> Path:
+    """Compile pic_oracle.c for the CPU this runs on (-march=native, the
+    Makefile's other flags) into ``dest``; used only by the benchmark's CPU
+    baseline, on the machine that times it. Returns the library's path."""
+    dest = Path(dest)
+    dest.mkdir(parents=True, exist_ok=True)
+    out = dest / "liboracle_native.so"
+    subprocess.run(["gcc", "-O3", "-march=native", "-ffp-contract=off", "-fno-fast-math", "-fopenmp", "-fPIC",
+                    "-std=c11", "-shared", "-o", str(out), str(HERE / "pic_oracle.c"), "-lm"], check=True)
+    return out
+
+
+def use_library(path) -> None:
+    """Load the oracle from ``path`` instead of the in-tree build (before the
+    first call)."""
+    global LIB_PATH, _LIB
+    LIB_PATH = Path(path)
+    _LIB = None
+
+
 def lib():
     global _LIB
     if _LIB is None:
@@ -64,6 +84,7 @@ def lib():
         L.orc_run.argtypes = [ctypes.POINTER(OrcParams), _DP, _DP, _IP, ctypes.c_int64, _DP, _DP, _DP, ctypes.c_int]
         L.orc_run.restype = ctypes.c_int
         L.orc_max_threads.restype = ctypes.c_int
+        L.orc_last_loop_seconds.restype = ctypes.c_double
         _LIB = L
     return _LIB
 
@@ -159,3 +180,9 @@ def run_batched(config, rng=None, n_steps=None, nthreads=0, fields=True):
 
 def max_threads() -> int:
     return int(lib().orc_max_threads())
+
+
+def last_loop_seconds() -> float:
+    """Wall time of the last run's step loop (allocation and first touch of
+    the arrays excluded)."""
+    return float(lib().orc_last_loop_seconds())

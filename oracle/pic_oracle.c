@@ -213,6 +213,16 @@ static void first_touch(const orc_params *q, double *a) {
  * Runs steps [0, n_steps). per_step is (n_steps, 4): injected, recombined,
  * total+, total-. pos_out/neg_out (may be NULL) receive the final current
  * arrays. Returns 0, or -1 if memory could not be allocated. */
+static double g_loop_seconds = 0.0;
+
+static double wall_seconds(void) {
+#ifdef _OPENMP
+    return omp_get_wtime();
+#else
+    return 0.0;
+#endif
+}
+
 int orc_run(const orc_params *q, const double *xs, const double *ys, const int64_t *offsets,
             int64_t n_steps, double *per_step, double *pos_out, double *neg_out, int nthreads) {
     set_threads(nthreads);
@@ -228,6 +238,7 @@ int orc_run(const orc_params *q, const double *xs, const double *ys, const int64
     }
     double *total = (double *)calloc(q->no_xy * q->no_xy, sizeof(double));
     double *pos = a[0], *neg = a[1], *pos_next = a[2], *neg_next = a[3];
+    const double t0 = wall_seconds();
     for (int64_t step = 0; step < n_steps; ++step) {
         const int64_t m = offsets[step + 1] - offsets[step];
         double injected = 0.0;
@@ -251,12 +262,17 @@ int orc_run(const orc_params *q, const double *xs, const double *ys, const int64
             tmp = neg; neg = neg_next; neg_next = tmp;
         }
     }
+    g_loop_seconds = wall_seconds() - t0;
     if (pos_out) memcpy(pos_out, pos, sizeof(double) * n);
     if (neg_out) memcpy(neg_out, neg, sizeof(double) * n);
     for (int s = 0; s < 4; ++s) free(a[s]);
     free(total);
     return 0;
 }
+
+/* Wall time of the last orc_run's step loop: allocation and first touch of
+ * the four arrays and the final copy-out excluded. */
+double orc_last_loop_seconds(void) { return g_loop_seconds; }
 
 int orc_max_threads(void) {
 #ifdef _OPENMP

@@ -14,7 +14,7 @@ import pytest
 from configs import SMALL, TWO_SPECIES, fast_config
 from oracle import adjoint as oracle_adjoint
 from pulsed_ion_chamber_hip import SimulationConfig
-from pulsed_ion_chamber_hip.adjoint import KsModel, ks_torch
+from pulsed_ion_chamber_hip.adjoint import PARAMS, KsModel, ks_torch
 from pulsed_ion_chamber_hip.solver_hip import HipSimulation
 
 pytestmark = pytest.mark.gpu
@@ -42,6 +42,26 @@ def test_gpu_gradient_matches_gpu_finite_differences():
     _, grad = model.ks_and_gradient()
     fd = model.finite_difference()
     np.testing.assert_allclose(grad, fd, rtol=2e-7)
+
+
+def test_gradient_on_three_pulses_matches_richardson_finite_differences():
+    """The adjoint benchmark configuration: 10 um voxels over r 0.02 cm
+    (48^2 x 206), three pulses (106 166 steps), FP64, checkpoints every 64
+    steps. All six derivatives agree with Richardson-extrapolated central
+    differences of the GPU forward run (relative steps 1e-3 and 2e-3) to
+    1e-6 relative; a plain quotient at step 1e-6 resolves only ~1e-4 here
+    (the rounding noise of k_s over 1e5 steps, divided by the step)."""
+    from configs import readme_config
+
+    cfg = readme_config(10.0, 0.02, n_pulses=3)
+    assert (cfg.no_xy, cfg.no_z_with_buffer, cfg.total_time_steps) == (48, 206, 106166)
+    model = KsModel(cfg, checkpoint_every=64)
+    ks, grad = model.ks_and_gradient()
+    fd = model.finite_difference_richardson(rel=1e-3)
+    rel = np.abs(grad - fd) / np.abs(fd)
+    print("3-pulse adjoint vs Richardson FD, relative error per parameter:", dict(zip(PARAMS, rel.tolist())))
+    assert ks == pytest.approx(model.ks(model.theta0), rel=1e-14)
+    assert np.all(rel <= 1e-6), rel
 
 
 def test_forward_at_theta0_is_the_plain_run():

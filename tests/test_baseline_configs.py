@@ -1,0 +1,202 @@
+"""Whole runs of every benchmark configuration on the GPU, and every run the
+reference publishes a full-precision result for, against that result.
+
+Pins, by configuration:
+
+* README example (10 um, r 0.008 cm, 24^2 x 206, 1622 steps) and the
+  mid-size solve (5 um, r 0.05 cm, 208^2 x 406, 3388 steps), library-default
+  physics, seed 1: the CPU oracle's k_s, f_t every 16 steps, summed injected
+  and recombined charge and checksums of the final fields
+  (tests/golden/oracle_runs.json; the README example is also rerun through the
+  oracle here and compared field by field).
+* The 32-point saturation curve as one batch: every point's k_s against the
+  oracle's.
+* The reference's published AIC-144 runs (profiling/data/gpu_athena/*.json,
+  profiling/data/gh200/convergence.json): 10 um ladder r 0.014 .. 0.265 cm,
+  r 0.05 cm at 5 and 2 um (506^2 x 1010, 13 019 steps), and the 5 um full
+  electrode (1066^2 x 410 = 466 M voxels, 4580 steps, 24.6 M tracks), k_s in
+  full precision.
+* The reference's 200-step prefix f at r 0.05 cm for 10, 5, 2 and 1 um
+  (profiling/data/gh200/resolution.json); 1 um is 1006^2 x 2010 = 2.03 G
+  voxels, 65 GB of FP64 fields.
+
+Bars: FP64 fields 1e-12 relative (k_s, f_t, sums, checksums; fields 1e-12 of
+the peak); FP32 fields |k_s - k_s(FP64)| <= 1e-6, the FP64 value being the
+published or oracle one. Track counts are exact. Each test prints the errors
+it measured.
+"""
+
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from configs import aic_config, readme_config, saturation_curve_configs
+from oracle import batched
+from pulsed_ion_chamber_hip import run_batch_hip, run_simulation_hip
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = Path(__file__).parent / "golden"
+PUBLISHED = json.loads((GOLDEN / "published.json").read_text())
+ORACLE = json.loads((GOLDEN / "oracle_runs.json").read_text())
+RTOL = 1e-12
+FP32_DKS = 1e-6
+
+
+def _rel(a, b):
+    return abs(a - b) / abs(b)
+
+
+def _check_against_summary(res, ref, fields=True):
+    cfg = res.config
+    assert [cfg.no_xy, cfg.no_xy, cfg.no_z_with_buffer] == ref["grid"]
+    assert int(res.track_density_xy.sum()) == ref["tracks"]
+    assert res.ks == pytest.approx(ref["ks"], rel=RTOL)
+    np.testing.assert_allclose(res.f_t[ref["f_t_index"]], ref["f_t"], rtol=RTOL)
+    assert float(np.sum(res.injected_positive)) == pytest.approx(ref["sum_injected"], rel=RTOL)
+    assert float(np.sum(res.recombination)) == pytest.approx(ref["sum_recombined"], rel=RTOL)
+    if fields:
+        for name, arr in (("positive", res.positive_array), ("negative", res.negative_array)):
+            got = {"sum": float(arr.sum()), "sum_sq": float(np.square(arr).sum()), "max": float(arr.max())}
+            for key, want in ref[name].items():
+                assert got[key] == pytest.approx(want, rel=RTOL), (name, key)
+
+
+# --- README example ---------------------------------------------------------
+
+
+def test_readme_example_fp64_matches_the_oracle_field_by_field():
+    cfg = readme_config(10.0, 0.008)
+    assert (cfg.no_xy, cfg.no_z_with_buffer, cfg.total_time_steps) == (24, 206, 1622)
+    ref = batched.run_batched(cfg)
+    gpu = run_simulation_hip(cfg, progress=False)
+    peak = np.abs(ref.positive_array).max()
+    err = max(np.abs(gpu.positive_array - ref.positive_array).max(),
+              np.abs(gpu.negative_array - ref.negative_array).max()) / peak
+    print(f"README example fp64: ks {gpu.ks!r} oracle {ref.ks!r} rel {_rel(gpu.ks, ref.ks):.2e} field {err:.2e}")
+    assert err <= RTOL
+    np.testing.assert_array_equal(gpu.track_density_xy, ref.track_density_xy)
+    np.testing.assert_allclose(gpu.f_t, ref.f_t, rtol=RTOL)
+    _check_against_summary(gpu, ORACLE["readme_10um_r0.008cm"])
+
+
+def test_readme_example_fp32_ks_within_1e6():
+    cfg = readme_config(10.0, 0.008)
+    gpu = run_simulation_hip(cfg, progress=False, precision="fp32", return_fields=False)
+    dks = abs(gpu.ks - ORACLE["readme_10um_r0.008cm"]["ks"])
+    print(f"README example fp32: |dks| {dks:.2e}")
+    assert dks <= FP32_DKS
+
+
+# --- mid-size solve ------------------------------------------------------------
+
+
+@pytest.fixture(scope="module")
+def mid_fp64():
+    return run_simulation_hip(readme_config(5.0, 0.05), progress=False)
+
+
+def test_mid_size_solve_fp64_matches_the_oracle(mid_fp64):
+    ref = ORACLE["mid_5um_r0.05cm"]
+    print(f"mid-size fp64: ks {mid_fp64.ks!r} oracle {ref['ks']!r} rel {_rel(mid_fp64.ks, ref['ks']):.2e}")
+    _check_against_summary(mid_fp64, ref)
+
+
+def test_mid_size_solve_fp32_ks_within_1e6(mid_fp64):
+    gpu = run_simulation_hip(readme_config(5.0, 0.05), progress=False, precision="fp32", return_fields=False)
+    dks = abs(gpu.ks - ORACLE["mid_5um_r0.05cm"]["ks"])
+    print(f"mid-size fp32: |dks| vs oracle {dks:.2e}, vs GPU fp64 {abs(gpu.ks - mid_fp64.ks):.2e}")
+    assert dks <= FP32_DKS
+    np.testing.assert_array_equal(gpu.track_density_xy, mid_fp64.track_density_xy)
+
+
+# --- saturation curve ------------------------------------------------------------
+
+
+def test_saturation_curve_batch_fp64_matches_the_oracle_at_every_point():
+    cfgs = saturation_curve_configs()
+    refs = ORACLE["saturation_curve"]
+    assert len(cfgs) == len(refs) == 32
+    got = run_batch_hip(cfgs, precision="fp64")
+    worst = max(_rel(res.ks, ref["ks"]) for res, ref in zip(got, refs))
+    print(f"curve fp64: max rel dks vs oracle {worst:.2e}")
+    for res, ref in zip(got, refs):
+        _check_against_summary(res, ref, fields=False)
+
+
+def test_saturation_curve_batch_fp32_ks_within_1e6_at_every_point():
+    cfgs = saturation_curve_configs()
+    got = run_batch_hip(cfgs, precision="fp32")
+    dks = [abs(res.ks - ref["ks"]) for res, ref in zip(got, ORACLE["saturation_curve"])]
+    print(f"curve fp32: max |dks| vs oracle {max(dks):.2e}")
+    assert max(dks) <= FP32_DKS
+
+
+# --- the reference's published runs ---------------------------------------------------------
+
+RUNS = [r for r in PUBLISHED["aic_runs"] if "gpu_ks" in r]
+RUN_IDS = [f"r{r['sampled_radius_cm']}cm@{r['grid_size_um']}um" for r in RUNS]
+# the 10 um r 0.05 cm run is listed twice (Athena ladder and GH200 convergence)
+_seen = set()
+UNIQUE = []
+for _r, _id in zip(RUNS, RUN_IDS):
+    if _id not in _seen:
+        _seen.add(_id)
+        UNIQUE.append((_r, _id))
+
+
+def _published_run(row, precision):
+    cfg = aic_config(row["sampled_radius_cm"], row["grid_size_um"])
+    assert (cfg.no_xy, cfg.no_z_with_buffer, cfg.total_time_steps) == (
+        row["no_xy"], row["no_z_with_buffer"], row["total_time_steps"])
+    return cfg, run_simulation_hip(cfg, progress=False, precision=precision, return_fields=False)
+
+
+_FP64_KS = {}
+
+
+@pytest.mark.parametrize("row", [r for r, _ in UNIQUE], ids=[i for _, i in UNIQUE])
+def test_published_ks_fp64(row):
+    cfg, res = _published_run(row, "fp64")
+    _FP64_KS[(row["sampled_radius_cm"], row["grid_size_um"])] = res.ks
+    msg = f"published r{row['sampled_radius_cm']}@{row['grid_size_um']}um fp64: ks {res.ks!r} gpu_ks {row['gpu_ks']!r}"
+    msg += f" rel {_rel(res.ks, row['gpu_ks']):.2e}"
+    if "cpu_ks" in row:
+        msg += f"; cpu_ks {row['cpu_ks']!r} rel {_rel(res.ks, row['cpu_ks']):.2e}"
+    print(msg)
+    assert res.ks == pytest.approx(row["gpu_ks"], rel=RTOL)
+    if "cpu_ks" in row:
+        assert res.ks == pytest.approx(row["cpu_ks"], rel=RTOL)
+    if "tracks_per_pulse" in row:
+        assert int(res.track_density_xy.sum()) == row["tracks_per_pulse"]
+    # the 200-step prefix f of the same run, where the reference publishes it
+    for pre in PUBLISHED["prefix_f"]:
+        if (pre["sampled_radius_cm"], pre["grid_size_um"]) == (row["sampled_radius_cm"], row["grid_size_um"]):
+            assert res.f_t[pre["steps"] - 1] == pytest.approx(pre["f_end"], rel=RTOL)
+
+
+@pytest.mark.parametrize("row", [r for r, _ in UNIQUE], ids=[i for _, i in UNIQUE])
+def test_published_ks_fp32_within_1e6(row):
+    cfg, res = _published_run(row, "fp32")
+    dks = abs(res.ks - row["gpu_ks"])
+    ks64 = _FP64_KS.get((row["sampled_radius_cm"], row["grid_size_um"]))
+    extra = "" if ks64 is None else f", vs GPU fp64 {abs(res.ks - ks64):.2e}"
+    print(f"published r{row['sampled_radius_cm']}@{row['grid_size_um']}um fp32: |dks| vs published {dks:.2e}{extra}")
+    assert dks <= FP32_DKS
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_one_micron_200_step_prefix(precision):
+    row = next(r for r in PUBLISHED["prefix_f"] if r["grid_size_um"] == 1.0)
+    cfg = aic_config(row["sampled_radius_cm"], 1.0, max_voxels=1e10)
+    assert (cfg.no_xy, cfg.no_z_with_buffer) == (row["no_xy"], row["no_z_with_buffer"])
+    res = run_simulation_hip(cfg, progress=False, precision=precision, max_steps=row["steps"], return_fields=False)
+    f = res.f_t[row["steps"] - 1]
+    print(f"1 um 200-step prefix {precision}: f {f!r} published {row['f_end']!r} abs {abs(f - row['f_end']):.2e}")
+    assert res.steps_completed == row["steps"]
+    if precision == "fp64":
+        assert f == pytest.approx(row["f_end"], rel=RTOL)
+    else:
+        assert abs(f - row["f_end"]) <= FP32_DKS

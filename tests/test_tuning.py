@@ -80,3 +80,24 @@ def test_default_tiling_follows_the_grid():
         peak = np.abs(out[0][0][:, col]).max()
         np.testing.assert_allclose(out[0][0][:, col], out[1][0][:, col], rtol=1e-10, atol=1e-12 * peak)
     assert out[0][0][:, 0].sum() > 0  # the steps deposited
+
+
+def test_cached_tiling_the_library_rejects_is_measured_again(tmp_path, monkeypatch):
+    """A cache entry this library does not accept (written by another build
+    under the same key) is replaced by a fresh measurement, not raised."""
+    monkeypatch.setenv("PIC_HIP_TUNING_CACHE", str(tmp_path / "tiling.json"))
+    cfg = SimulationConfig(**{**SMALL, "grid_size_um": 20.0})
+    key = tuning.cache_key(cfg, "fp64")
+    (tmp_path / "tiling.json").write_text(json.dumps({key: {"tiling": [3, 4, 1], "sweep_us": 1.0}}))
+    res = run_simulation_hip(cfg, progress=False, tiling="auto")
+    plain = run_simulation_hip(cfg, progress=False)
+    np.testing.assert_array_equal(res.positive_array, plain.positive_array)
+    entry = json.loads((tmp_path / "tiling.json").read_text())[key]
+    assert tuple(entry["tiling"]) in tuning.candidates("fp64")
+
+
+def test_cache_key_names_the_library_build():
+    from pulsed_ion_chamber_hip import _lib
+
+    cfg = SimulationConfig(**SMALL)
+    assert f"lib {_lib.library_digest()}" in tuning.cache_key(cfg, "fp64")
