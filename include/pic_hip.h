@@ -107,6 +107,24 @@ int pic_read_per_step(pic_sim *sim, int64_t first, int64_t count, double *out);
 int pic_run_pulse(pic_sim *sim, const double *xs, const double *ys, const int64_t *step_offsets, int64_t n_steps,
                   double *per_step_out);
 
+/* pic_run_pulse without any allocation, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, hosts
+ * that forbid hipMalloc). It has no counterpart in the reference. Same
+ * arguments, results and errors as pic_run_pulse, bit for bit, except:
+ *  - the device copies of the offsets and track centres and the device
+ *    per-step record are staged in a static device array of 64 MiB that the
+ *    library holds on every GPU it runs on, so a call fails with
+ *    PIC_E_INVALID when 32*n_steps + 8*(n_steps+1) + 16*n_tracks bytes
+ *    (each region rounded up to 256) exceed 64 MiB -- about four million
+ *    tracks;
+ *  - the handle's uploaded tracks (pic_upload_tracks) are left as they were;
+ *  - launches are direct: graph replay and timing do not apply.
+ * The density slots and partials are the handle's own, allocated by
+ * pic_create. Every call on one GPU shares that GPU's array: calls that
+ * overlap in time on the same GPU (two threads, or two handles) race. */
+int pic_run_pulse_static(pic_sim *sim, const double *xs, const double *ys, const int64_t *step_offsets,
+                         int64_t n_steps, double *per_step_out);
+
 /* Kernel-level calls (the granularity tests/test_charge_conservation.py
  * drives _insert_track_numba / _lax_wendroff_step_numba at):
  * deposit m tracks into the current arrays and return the scored injected

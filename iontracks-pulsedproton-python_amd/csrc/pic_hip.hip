@@ -26,6 +26,15 @@
 #include "pic_kernels.hpp"
 #include "pic_adjoint.hpp"
 
+// Staging of pic_run_pulse_static, the entry point that allocates nothing:
+// the per-step record, the step offsets and the track centres of one call.
+// Every GPU the library runs on holds its own copy for the life of the
+// process. In host code `scratch` names a host-side stand-in, not device
+// memory: kernels and copies get the per-device address from
+// static_scratch() below, never the array's name.
+constexpr size_t kStaticScratchBytes = 64ull << 20;
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
 namespace {
 
 thread_local char g_err[1024] = "";
@@ -927,6 +936,28 @@ int run_lanes(pic_batch *b, const std::vector<int> &part, int64_t first, int64_t
     return PIC_OK;
 }
 
+// --- static scratch --------------------------------------------------
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char *g_scratch_dev[kMaxDevices] = {};  // device address of `scratch`, per device
+
+// The device address of `scratch` on `device` (which must be current):
+// looked up once per device with hipGetSymbolAddress, then reused.
+int static_scratch(int device, unsigned char **out) {
+    if (device < 0 || device >= kMaxDevices) return fail(PIC_E_INVALID, "device %d outside 0..%d", device, kMaxDevices - 1);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[device]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(PIC_E_HIP, "hipGetSymbolAddress of the static scratch failed: %s", hipGetErrorString(e));
+        g_scratch_dev[device] = static_cast<unsigned char *>(ptr);
+    }
+    *out = g_scratch_dev[device];
+    return PIC_OK;
+}
+
 // --- adjoint helpers -------------------------------------------------
 
 // i planes per adjoint block: as few as keep the grid within one wave of
@@ -1202,32 +1233,57 @@ int pic_upload_tracks(pic_sim *s, const double *xs, const double *ys, int64_t n_
     return PIC_OK;
 }
 
-int pic_run_steps(pic_sim *s, int64_t first, int64_t count) {
-    if (!s) return fail(PIC_E_INVALID, "sim is NULL");
-    if (first < 0 || count < 0 || first + count > s->n_steps)
-        return fail(PIC_E_INVALID, "steps [%lld, %lld) outside the uploaded %lld", (long long)first,
-                    (long long)(first + count), (long long)s->n_steps);
+}  // extern "C"
+
+namespace {
+
+// The tracks and per-step record a run works on: the handle's uploaded
+// arrays (pic_run_steps) or the static scratch (pic_run_pulse_static).
+struct TrackView {
+    const double *xs = nullptr, *ys = nullptr;
+    const int64_t *offsets_d = nullptr;  // device step offsets, n_steps + 1
+    const int64_t *offsets_h = nullptr;  // the same on the host
+    double *per_step = nullptr;          // device (n_steps, 4) record
+    int64_t n_steps = 0;
+};
+
+TrackView uploaded_tracks(const pic_sim *s) {
+    TrackView t;
+    t.xs = s->xs;
+    t.ys = s->ys;
+    t.offsets_d = s->offsets_d;
+    t.offsets_h = s->offsets.data();
+    t.per_step = s->per_step;
+    t.n_steps = s->n_steps;
+    return t;
+}
+
+// Steps [first, first+count) over the tracks of `tv`. `direct`: launch one
+// kernel at a time with no timing events and no graph replay (no host or
+// device memory is allocated on this path).
+int run_steps_on(pic_sim *s, const TrackView &tv, int64_t first, int64_t count, bool direct) {
     HIP_TRY(hipStreamSynchronize(s->stream));  // events of an earlier call may still be pending
     HIP_TRY(hipStreamSynchronize(s->gstream));
     s->ev_used = 0;
     s->group_open = 0;
-    s->in_run = true;
+    s->in_run = !direct;  // timing events are recorded only inside a run that asks for them
     int rc = PIC_OK;
     const int n_tiles = (int)gather_blocks(s);
     const int64_t xy = s->p.no_xy * s->p.no_xy;
     const int64_t slots = dens_slots(s->batch);
     const int64_t end = first + count;
+    const int64_t *off = tv.offsets_h;
     // One segment per gather batch. Launched one by one, a segment's gather
     // runs on the side stream beside the previous segment's sweeps: it starts
     // once that segment's first sweep has folded the step before it, and the
     // segment's sweeps wait for it. With graph replay a segment's gather and
     // sweeps are collected, then replayed through graph chains in order.
-    const bool chained = s->graphs && !s->timing;
+    const bool chained = s->graphs && !s->timing && !direct;
     // the gather of segment [g0, ...) on the side stream: 1 queued, 0 no tracks, else an error
     auto side_gather = [&](int64_t g0) -> int {
         const int64_t g1 = std::min(end, g0 + s->batch);
-        if (g0 >= end || s->offsets[g1] <= s->offsets[g0]) return 0;
-        int r = enqueue_gather(s, s->xs, s->ys, s->offsets_d, g0, g1 - g0, s->gstream);
+        if (g0 >= end || off[g1] <= off[g0]) return 0;
+        int r = enqueue_gather(s, tv.xs, tv.ys, tv.offsets_d, g0, g1 - g0, s->gstream);
         if (!r && hipEventRecord(s->seg_dens, s->gstream) != hipSuccess) r = fail(PIC_E_HIP, "hipEventRecord failed");
         return r ? r : 1;
     };
@@ -1242,16 +1298,16 @@ int pic_run_steps(pic_sim *s, int64_t first, int64_t count) {
         if (chained) {
             s->seg.clear();
             s->sink = &s->seg;
-            if (s->offsets[b1] > s->offsets[b0]) rc = enqueue_gather(s, s->xs, s->ys, s->offsets_d, b0, b1 - b0);
+            if (off[b1] > off[b0]) rc = enqueue_gather(s, tv.xs, tv.ys, tv.offsets_d, b0, b1 - b0);
         } else if (gather_queued) {
             if (hipStreamWaitEvent(s->stream, s->seg_dens, 0) != hipSuccess) rc = fail(PIC_E_HIP, "hipStreamWaitEvent failed");
             gather_queued = false;
         }
         for (int64_t step = b0; step < b1 && rc == PIC_OK; ++step) {
-            const bool dep = s->offsets[step + 1] > s->offsets[step];
+            const bool dep = off[step + 1] > off[step];
             const int64_t slot = step % slots;
             rc = enqueue_sweep(s, step, dep, s->dens + slot * xy, s->dep_part + slot * n_tiles, n_tiles,
-                               s->per_step + 4 * step);
+                               tv.per_step + 4 * step);
             if (!rc) swap_buffers(s);
             if (!rc && !chained && step == b0 && b1 < end) {
                 if (hipEventRecord(s->seg_go, s->stream) != hipSuccess ||
@@ -1281,6 +1337,18 @@ int pic_run_steps(pic_sim *s, int64_t first, int64_t count) {
     return rc;
 }
 
+}  // namespace
+
+extern "C" {
+
+int pic_run_steps(pic_sim *s, int64_t first, int64_t count) {
+    if (!s) return fail(PIC_E_INVALID, "sim is NULL");
+    if (first < 0 || count < 0 || first + count > s->n_steps)
+        return fail(PIC_E_INVALID, "steps [%lld, %lld) outside the uploaded %lld", (long long)first,
+                    (long long)(first + count), (long long)s->n_steps);
+    return run_steps_on(s, uploaded_tracks(s), first, count, false);
+}
+
 int pic_read_per_step(pic_sim *s, int64_t first, int64_t count, double *out) {
     if (!s || !out) return fail(PIC_E_INVALID, "NULL argument");
     if (first < 0 || count < 0 || first + count > s->n_steps) return fail(PIC_E_INVALID, "rows outside the record");
@@ -1298,6 +1366,58 @@ int pic_run_pulse(pic_sim *s, const double *xs, const double *ys, const int64_t 
     if (!rc) rc = pic_run_steps(s, 0, n_steps);
     if (!rc) rc = pic_read_per_step(s, 0, n_steps, per_step_out);
     return rc;
+}
+
+int pic_run_pulse_static(pic_sim *s, const double *xs, const double *ys, const int64_t *step_offsets,
+                         int64_t n_steps, double *per_step_out) {
+    if (!s || !step_offsets || !per_step_out) return fail(PIC_E_INVALID, "NULL argument");
+    if (n_steps < 1) return fail(PIC_E_INVALID, "n_steps must be positive");
+    const int64_t n_tracks = step_offsets[n_steps];
+    if (step_offsets[0] != 0 || n_tracks < 0) return fail(PIC_E_INVALID, "offsets must run from 0 to n_tracks");
+    for (int64_t i = 0; i < n_steps; ++i)
+        if (step_offsets[i + 1] < step_offsets[i]) return fail(PIC_E_INVALID, "offsets must be non-decreasing");
+    if (n_tracks > 0 && (!xs || !ys)) return fail(PIC_E_INVALID, "xs/ys NULL with tracks in the schedule");
+    // regions of the scratch, each 256-byte aligned: record, offsets, xs, ys
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t rec = up((size_t)n_steps * 32), offb = up((size_t)(n_steps + 1) * 8), trk = up((size_t)n_tracks * 8);
+    const size_t need = rec + offb + 2 * trk;
+    if (need > kStaticScratchBytes)
+        return fail(PIC_E_INVALID, "%lld steps and %lld tracks need %.2f MiB of staging; the static scratch of "
+                    "pic_run_pulse_static holds %zu MiB (pic_run_pulse has no such limit)", (long long)n_steps,
+                    (long long)n_tracks, need / 1048576.0, kStaticScratchBytes >> 20);
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev != s->device) return fail(PIC_E_INVALID, "the handle lives on device %d, current is %d", s->device, dev);
+    unsigned char *base = nullptr;
+    int rc = static_scratch(dev, &base);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    HIP_TRY(hipStreamSynchronize(s->gstream));
+    TrackView tv;
+    tv.per_step = reinterpret_cast<double *>(base);
+    int64_t *offsets_d = reinterpret_cast<int64_t *>(base + rec);
+    double *xs_d = reinterpret_cast<double *>(base + rec + offb);
+    double *ys_d = reinterpret_cast<double *>(base + rec + offb + trk);
+    tv.offsets_d = offsets_d;
+    tv.offsets_h = step_offsets;
+    tv.xs = xs_d;
+    tv.ys = ys_d;
+    tv.n_steps = n_steps;
+    HIP_TRY(hipMemcpyAsync(offsets_d, step_offsets, (size_t)(n_steps + 1) * 8, hipMemcpyHostToDevice, s->stream));
+    if (n_tracks > 0) {
+        HIP_TRY(hipMemcpyAsync(xs_d, xs, (size_t)n_tracks * 8, hipMemcpyHostToDevice, s->stream));
+        HIP_TRY(hipMemcpyAsync(ys_d, ys, (size_t)n_tracks * 8, hipMemcpyHostToDevice, s->stream));
+    }
+    // Nothing here must start at zero: every row of the record is written
+    // whole by the fold of its step. The gathers read the offsets and centres
+    // only after the copies above (same stream; the side stream's first
+    // gather is queued inside run_steps_on after it synchronises both).
+    s->pending = pic_sim::Pending{};
+    rc = run_steps_on(s, tv, 0, n_steps, true);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(per_step_out, tv.per_step, (size_t)n_steps * 32, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return PIC_OK;
 }
 
 int pic_deposit(pic_sim *s, const double *xs, const double *ys, int64_t m, double *inserted) {

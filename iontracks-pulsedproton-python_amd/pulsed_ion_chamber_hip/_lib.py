@@ -36,6 +36,7 @@ EXPORTED = (
     "pic_run_steps",
     "pic_read_per_step",
     "pic_run_pulse",
+    "pic_run_pulse_static",
     "pic_deposit",
     "pic_step",
     "pic_read_fields",
@@ -125,6 +126,7 @@ def open_library():
         "pic_run_steps": [_VP, ctypes.c_int64, ctypes.c_int64],
         "pic_read_per_step": [_VP, ctypes.c_int64, ctypes.c_int64, _DP],
         "pic_run_pulse": [_VP, _DP, _DP, ctypes.POINTER(ctypes.c_int64), ctypes.c_int64, _DP],
+        "pic_run_pulse_static": [_VP, _DP, _DP, ctypes.POINTER(ctypes.c_int64), ctypes.c_int64, _DP],
         "pic_deposit": [_VP, _DP, _DP, ctypes.c_int64, _DP],
         "pic_step": [_VP, _DP],
         "pic_read_fields": [_VP, _DP, _DP],

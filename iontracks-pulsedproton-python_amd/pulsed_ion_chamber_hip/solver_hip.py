@@ -115,6 +115,21 @@ class HipSimulation:
         self.n_steps = len(schedule)
         return out
 
+    def run_pulse_static(self, schedule: np.ndarray, xs: np.ndarray, ys: np.ndarray) -> np.ndarray:
+        """As :meth:`run_pulse`, through the entry point that allocates nothing
+        (pic_run_pulse_static: inputs staged in the library's static 64 MiB
+        device array; the handle's uploaded tracks are left untouched)."""
+        offsets = np.zeros(len(schedule) + 1, dtype=np.int64)
+        np.cumsum(schedule, out=offsets[1:])
+        xs = np.ascontiguousarray(xs, dtype=np.float64)
+        ys = np.ascontiguousarray(ys, dtype=np.float64)
+        if len(xs) != offsets[-1] or len(ys) != offsets[-1]:
+            raise ValueError("xs/ys length does not match the schedule total")
+        out = np.zeros((len(schedule), 4))
+        _lib.check(self.L.pic_run_pulse_static(self.handle, _lib.dp(xs), _lib.dp(ys), _lib.ip(offsets),
+                                               len(schedule), _lib.dp(out)), "pic_run_pulse_static")
+        return out
+
     def run_steps(self, first: int, count: int) -> None:
         _lib.check(self.L.pic_run_steps(self.handle, int(first), int(count)), "pic_run_steps")
 
