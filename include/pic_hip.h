@@ -41,6 +41,10 @@ extern "C" {
 #define PIC_WALL_ABSORBING 0
 #define PIC_WALL_REFLECTING 1
 
+#define PIC_GATHER_AUTO 0   /* bin tracks by tile on grids of >= 256 gather tiles */
+#define PIC_GATHER_SCAN 1   /* every tile tests every track of its step */
+#define PIC_GATHER_BINNED 2 /* tracks binned by tile first, on any grid */
+
 typedef struct pic_sim pic_sim;
 typedef struct pic_batch pic_batch;
 typedef struct pic_adjoint pic_adjoint;
@@ -144,6 +148,17 @@ int pic_write_fields(pic_sim *sim, const double *pos, const double *neg);
 /* Zero the four carrier arrays and the per-step record, keeping the uploaded
  * tracks: the state of a fresh handle (a rerun of the same schedule). */
 int pic_reset(pic_sim *sim);
+
+/* How the deposition finds the tracks that reach each 16x16 tile of columns
+ * (the role of the CSR row index of solver_numba_parallel.py:140-180): every
+ * tile testing every track of its step, or the tracks binned by tile on the
+ * device first (count, scan, fill), each tile then sorting and reading only
+ * its own bin. Both add the same terms in the same ascending track order, so
+ * the results are bit-identical; binning is faster on large grids. The mode
+ * applies to the uploaded tracks (set before or after pic_upload_tracks);
+ * get_gather_mode reports whether bins are in use. */
+int pic_set_gather_mode(pic_sim *sim, int mode);
+int pic_get_gather_mode(pic_sim *sim, int *binned);
 
 /* Block until all work queued on the handle's stream is done. */
 int pic_synchronize(pic_sim *sim);

@@ -138,6 +138,13 @@ struct pic_sim {
     bool graphs = false;  // replay a run's launches through graph chains (never while timing)
     std::vector<KLaunch> *sink = nullptr;  // open segment: launches are collected here
     std::vector<KLaunch> seg;              // a run's segment buffer, reused
+    // Tracks binned by gather tile for the uploaded run (pic_set_gather_mode):
+    // counters, scans and cursors per (step of a segment, tile), entries per
+    // step, and the entry list, sized for the busiest segment of the upload.
+    int gather_mode = PIC_GATHER_AUTO;
+    bool bins_ready = false;
+    int *bin_cnt = nullptr, *bin_off = nullptr, *bin_cur = nullptr, *bin_tot = nullptr, *bin_list = nullptr;
+    int64_t bin_cap = 0;
 };
 
 // Many configurations advanced together: per global step one sweep launch
@@ -709,13 +716,114 @@ pic::GatherArgs make_gather_args(const pic_sim *s, const double *xs, const doubl
 }
 
 // deposition densities of steps [first, first+count) (count <= batch) into
-// their dens and dep_part slots
+// their dens and dep_part slots. With `offsets_h` (the host copy of the
+// step offsets) and the handle's bins ready, the tracks are first binned by
+// tile (clear, count, scan, fill) on the same stream, and each gather block
+// reads only its bin; otherwise every block scans its step's tracks. The
+// results are the same bit for bit.
 int enqueue_gather(pic_sim *s, const double *xs, const double *ys, const int64_t *offsets, int64_t first,
-                   int64_t count, hipStream_t st = nullptr) {
-    const pic::GatherArgs g = make_gather_args(s, xs, ys, offsets, first);
+                   int64_t count, hipStream_t st = nullptr, const int64_t *offsets_h = nullptr) {
+    st = st ? st : s->stream;
+    pic::GatherArgs g = make_gather_args(s, xs, ys, offsets, first);
+    if (offsets_h && s->bins_ready) {
+        const int64_t tiles = gather_blocks(s);
+        int64_t most = 0;
+        for (int64_t t = first; t < first + count; ++t) most = std::max(most, offsets_h[t + 1] - offsets_h[t]);
+        pic::BinArgs b{};
+        b.xs = xs;
+        b.ys = ys;
+        b.offsets = offsets;
+        b.first_step = first;
+        b.cnt = s->bin_cnt;
+        b.off = s->bin_off;
+        b.cur = s->bin_cur;
+        b.tot = s->bin_tot;
+        b.list = s->bin_list;
+        b.n_cnt = count * tiles;
+        b.no_xy = (int)s->p.no_xy;
+        b.ntj = (int)ceil_div(s->p.no_xy, pic::kGT);
+        b.cutoff = s->p.cutoff_voxels;
+        const unsigned clear_blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(b.n_cnt, 256), 1024));
+        const dim3 per_track((unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(most, 256), 64)), (unsigned)count);
+        const KLaunch passes[] = {
+            KLaunch::make((const void *)pic::bin_clear_kernel, dim3(clear_blocks), dim3(256), 1, b),
+            KLaunch::make((const void *)pic::bin_count_kernel, per_track, dim3(256), 1, b),
+            KLaunch::make((const void *)pic::bin_scan_kernel, dim3((unsigned)count), dim3(256), 1, b),
+            KLaunch::make((const void *)pic::bin_fill_kernel, per_track, dim3(256), 1, b),
+        };
+        for (const KLaunch &l : passes) {
+            const int rc = emit(s, l, st);
+            if (rc) return rc;
+        }
+        g.bins.cnt = s->bin_cnt;
+        g.bins.off = s->bin_off;
+        g.bins.tot = s->bin_tot;
+        g.bins.list = s->bin_list;
+    }
     return emit(s, KLaunch::make((const void *)pic::gather_kernel, dim3((unsigned)gather_blocks(s), (unsigned)count),
                                  dim3(256), 1, g),
-                st ? st : s->stream);
+                st);
+}
+
+// Tiles a track's stencil box can meet, at most: a box spans at most
+// floor(2 cutoff) + 1 cells per axis, hence (w - 1) / 16 + 2 tiles.
+int64_t bin_tiles_per_track(const pic_sim *s) {
+    const int64_t ntj = ceil_div(s->p.no_xy, pic::kGT);
+    const double w = std::floor(2.0 * s->p.cutoff_voxels) + 1.0;
+    const int64_t axis = w >= (double)(ntj * pic::kGT) ? ntj : std::min<int64_t>(ntj, ((int64_t)w - 1) / pic::kGT + 2);
+    return axis * axis;
+}
+
+void free_bins(pic_sim *s) {
+    int **bufs[] = {&s->bin_cnt, &s->bin_off, &s->bin_cur, &s->bin_tot, &s->bin_list};
+    for (int **b : bufs) {
+        if (*b) hipFree(*b);
+        *b = nullptr;
+    }
+    s->bins_ready = false;
+    s->bin_cap = 0;
+}
+
+// Size the bins for the uploaded tracks when the gather mode asks for them
+// (automatic: grids of at least 256 gather tiles, where scanning every track
+// per tile costs most -- 4489 tiles x 6646 tracks per step on the 5 um full
+// electrode). Capacity: the most entries any window of `batch` consecutive
+// steps can hold. A list that would pass 1 GB (a huge cutoff) keeps the scan.
+int ensure_bins(pic_sim *s) {
+    free_bins(s);
+    if (s->gather_mode == PIC_GATHER_SCAN || s->n_steps <= 0) return PIC_OK;
+    const int64_t tiles = gather_blocks(s);
+    if (s->gather_mode == PIC_GATHER_AUTO && tiles < 256) return PIC_OK;
+    int64_t most = 0;
+    for (int64_t t = 0; t < s->n_steps; ++t)
+        most = std::max(most, s->offsets[std::min(t + s->batch, s->n_steps)] - s->offsets[t]);
+    const int64_t per = bin_tiles_per_track(s);
+    const int64_t cap = std::max<int64_t>(1, most) * per;
+    if (most > INT_MAX / std::max<int64_t>(per, 1) || cap * 4 > (1LL << 30)) {
+        if (s->gather_mode == PIC_GATHER_BINNED)
+            return fail(PIC_E_INVALID, "binned deposition would need %lld list entries (over 1 GB); use the scan",
+                        (long long)cap);
+        return PIC_OK;
+    }
+    const size_t counters = (size_t)s->batch * tiles * sizeof(int);
+    const size_t need = 3 * counters + (size_t)s->batch * sizeof(int) + (size_t)cap * sizeof(int);
+    size_t fr = 0, tot = 0;
+    HIP_TRY(hipMemGetInfo(&fr, &tot));
+    if ((double)need > 0.9 * (double)fr) {
+        if (s->gather_mode == PIC_GATHER_BINNED)
+            return fail(PIC_E_NOMEM, "binned deposition needs %.3f GiB, %.3f GiB free", need / 1073741824.0,
+                        fr / 1073741824.0);
+        return PIC_OK;
+    }
+    if (hipMalloc(&s->bin_cnt, counters) != hipSuccess || hipMalloc(&s->bin_off, counters) != hipSuccess ||
+        hipMalloc(&s->bin_cur, counters) != hipSuccess || hipMalloc(&s->bin_tot, (size_t)s->batch * sizeof(int)) != hipSuccess ||
+        hipMalloc(&s->bin_list, (size_t)cap * sizeof(int)) != hipSuccess) {
+        free_bins(s);
+        return fail(PIC_E_NOMEM, "hipMalloc of the deposition bins failed");
+    }
+    s->bin_cap = cap;
+    s->bins_ready = true;
+    return PIC_OK;
 }
 
 int enqueue_deposit_reduce(pic_sim *s, double *out4) {
@@ -1180,6 +1288,7 @@ int pic_destroy(pic_sim *s) {
                     s->scratch4, s->per_step, s->xs, s->ys, s->offsets_d};
     for (void *b : bufs)
         if (b) hipFree(b);
+    free_bins(s);
     for (hipEvent_t e : s->ev_pool) hipEventDestroy(e);
     if (s->seg_go) hipEventDestroy(s->seg_go);
     if (s->seg_dens) hipEventDestroy(s->seg_dens);
@@ -1200,6 +1309,7 @@ int pic_upload_tracks(pic_sim *s, const double *xs, const double *ys, int64_t n_
     HIP_TRY(hipStreamSynchronize(s->stream));
     HIP_TRY(hipStreamSynchronize(s->gstream));  // a side gather reads the track arrays
     s->pending = pic_sim::Pending{};
+    free_bins(s);
     if (s->xs) hipFree(s->xs);
     if (s->ys) hipFree(s->ys);
     if (s->per_step) hipFree(s->per_step);
@@ -1230,6 +1340,28 @@ int pic_upload_tracks(pic_sim *s, const double *xs, const double *ys, int64_t n_
     s->offsets.assign(offsets, offsets + n_steps + 1);
     s->n_tracks = n_tracks;
     s->n_steps = n_steps;
+    return ensure_bins(s);
+}
+
+int pic_set_gather_mode(pic_sim *s, int mode) {
+    if (!s) return fail(PIC_E_INVALID, "sim is NULL");
+    if (mode != PIC_GATHER_AUTO && mode != PIC_GATHER_SCAN && mode != PIC_GATHER_BINNED)
+        return fail(PIC_E_INVALID, "gather mode must be PIC_GATHER_AUTO, PIC_GATHER_SCAN or PIC_GATHER_BINNED");
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    HIP_TRY(hipStreamSynchronize(s->gstream));
+    const int old = s->gather_mode;
+    s->gather_mode = mode;
+    const int rc = ensure_bins(s);
+    if (rc) {
+        s->gather_mode = old;
+        ensure_bins(s);
+    }
+    return rc;
+}
+
+int pic_get_gather_mode(pic_sim *s, int *binned) {
+    if (!s || !binned) return fail(PIC_E_INVALID, "NULL argument");
+    *binned = s->bins_ready ? 1 : 0;
     return PIC_OK;
 }
 
@@ -1245,6 +1377,7 @@ struct TrackView {
     const int64_t *offsets_h = nullptr;  // the same on the host
     double *per_step = nullptr;          // device (n_steps, 4) record
     int64_t n_steps = 0;
+    bool uploaded = false;               // the handle's upload, whose bins the handle holds
 };
 
 TrackView uploaded_tracks(const pic_sim *s) {
@@ -1255,6 +1388,7 @@ TrackView uploaded_tracks(const pic_sim *s) {
     t.offsets_h = s->offsets.data();
     t.per_step = s->per_step;
     t.n_steps = s->n_steps;
+    t.uploaded = true;
     return t;
 }
 
@@ -1283,7 +1417,7 @@ int run_steps_on(pic_sim *s, const TrackView &tv, int64_t first, int64_t count, 
     auto side_gather = [&](int64_t g0) -> int {
         const int64_t g1 = std::min(end, g0 + s->batch);
         if (g0 >= end || off[g1] <= off[g0]) return 0;
-        int r = enqueue_gather(s, tv.xs, tv.ys, tv.offsets_d, g0, g1 - g0, s->gstream);
+        int r = enqueue_gather(s, tv.xs, tv.ys, tv.offsets_d, g0, g1 - g0, s->gstream, tv.uploaded ? off : nullptr);
         if (!r && hipEventRecord(s->seg_dens, s->gstream) != hipSuccess) r = fail(PIC_E_HIP, "hipEventRecord failed");
         return r ? r : 1;
     };
@@ -1298,7 +1432,8 @@ int run_steps_on(pic_sim *s, const TrackView &tv, int64_t first, int64_t count, 
         if (chained) {
             s->seg.clear();
             s->sink = &s->seg;
-            if (off[b1] > off[b0]) rc = enqueue_gather(s, tv.xs, tv.ys, tv.offsets_d, b0, b1 - b0);
+            if (off[b1] > off[b0])
+                rc = enqueue_gather(s, tv.xs, tv.ys, tv.offsets_d, b0, b1 - b0, s->stream, tv.uploaded ? off : nullptr);
         } else if (gather_queued) {
             if (hipStreamWaitEvent(s->stream, s->seg_dens, 0) != hipSuccess) rc = fail(PIC_E_HIP, "hipStreamWaitEvent failed");
             gather_queued = false;
@@ -2021,7 +2156,7 @@ int pic_adjoint_run(pic_adjoint *ad, double sums[2], double d_rec[11], double d_
         const int64_t s0 = k * K, e = std::min(T, s0 + K);
         HIP_TRY(hipMemcpyAsync(ad->store, ad->ckpt + k * 2 * nv, 2 * nv * 8, hipMemcpyDeviceToDevice, s->stream));
         if (s->offsets[e] > s->offsets[s0]) {
-            rc = enqueue_gather(s, s->xs, s->ys, s->offsets_d, s0, e - s0);
+            rc = enqueue_gather(s, s->xs, s->ys, s->offsets_d, s0, e - s0, s->stream, s->offsets.data());
             if (!rc) rc = adjoint_deriv_gather(ad, s0, e - s0);
         }
         for (int64_t t = s0; t + 1 < e && rc == PIC_OK; ++t) rc = adjoint_recompute_sweep(ad, t, t - s0);

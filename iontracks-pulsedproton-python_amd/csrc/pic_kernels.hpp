@@ -11,6 +11,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <climits>
+
 #include <type_traits>
 #include <utility>
 
@@ -737,6 +739,20 @@ __global__ void __launch_bounds__(512) batch_sweep_kernel(BatchTable<SweepArgs> 
     sweep_body<S, A, TJ, V, REFLECT>(a, g - first, true);
 }
 
+// Tracks binned by gather tile, for one launch's steps (built on the device
+// by bin_count_kernel, bin_scan_kernel and bin_fill_kernel, in that order on
+// one stream): per (step y of the launch, tile) the number of tracks whose
+// stencil box meets the tile, `cnt`, and where their entries start among the
+// step's, `off`; per step its number of entries, `tot`; the entries are the
+// tracks' indices within their step, in `list` (each bin in no particular
+// order: the gather sorts it).
+struct BinView {
+    const int *cnt;   // [steps][tiles]
+    const int *off;   // [steps][tiles]
+    const int *tot;   // [steps]
+    const int *list;  // entries of step y start at sum(tot[0..y))
+};
+
 struct GatherArgs {
     const double *xs;
     const double *ys;
@@ -751,10 +767,14 @@ struct GatherArgs {
     // derivatives with respect to gf and h2b2 ([gridDim.y][no_xy^2] each) and
     // the per-block scored injected-charge derivatives ([gridDim.y][gridDim.x][2])
     double *dgf, *dh, *dpart;
+    // binned tracks of this launch's steps (gather_kernel only), or all null:
+    // then every block scans all of its step's tracks
+    BinView bins;
 };
 
 constexpr int kGT = 16;  // gather tile edge
 constexpr int kGB = 128;  // tracks whose factors are evaluated per LDS batch
+constexpr int kBinCap = 2048;  // largest bin a block sorts in LDS; larger bins fall back to the scan
 
 __device__ __forceinline__ void track_box(double x, double y, double c, double nx, int &li, int &hi, int &lj,
                                           int &hj) {
@@ -764,18 +784,56 @@ __device__ __forceinline__ void track_box(double x, double y, double c, double n
     hj = (int)fmin(floor(y + c) + 1.0, nx);
 }
 
+// The gather tiles a track's stencil box meets, [ti0, ti1] x [tj0, tj1];
+// false for an empty box (which adds nothing anywhere). A tile meets the box
+// exactly when the scan's test in gather_body says so.
+__device__ __forceinline__ bool track_tiles(double x, double y, double c, double nx, int &ti0, int &ti1, int &tj0,
+                                            int &tj1) {
+    int li, hi, lj, hj;
+    track_box(x, y, c, nx, li, hi, lj, hj);
+    if (hi <= li || hj <= lj) return false;
+    ti0 = li / kGT;
+    ti1 = (hi - 1) / kGT;
+    tj0 = lj / kGT;
+    tj1 = (hj - 1) / kGT;
+    return true;
+}
+
+// Sum of tot[0..y) (y <= a launch's steps, at most a few hundred): the first
+// entry of step y in the bin list. Every thread of the block returns it.
+__device__ __forceinline__ int bin_step_base(const int *tot, int y) {
+    __shared__ int s_base;
+    if (threadIdx.x < kWave) {
+        int v = 0;
+        for (int q = threadIdx.x; q < y; q += kWave) v += tot[q];
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, kWave);
+        if (threadIdx.x == 0) s_base = v;
+    }
+    __syncthreads();
+    return s_base;
+}
+
 // Deposition of a block of steps as a gather. Block (x, y) owns a 16x16 tile
 // of columns for step first_step + y; each thread owns one (i, j) column and
 // sums, in ascending track order, gi(t)*gj(t) over the tracks whose stencil
 // box covers it -- the same terms in the same order as the reference's CSR
 // row accumulation (solver_numba_parallel.py:183-219), so no atomics and a
-// deterministic result. The block first compacts, in order, the tracks whose
-// box meets the tile (ballot + prefix count), then evaluates each listed
-// track's 16 row and 16 column factors once into LDS, 128 tracks at a time
-// (few barriers, and sixteen independent exp evaluations per thread). A
-// factor whose row or column lies outside the track's box is stored as zero,
-// so the sum needs no box test: a skipped term and an added +0.0 leave the
-// (non-negative) sum bit-identical.
+// deterministic result.
+//
+// The block's list of tracks -- those whose box meets the tile, ascending --
+// comes from one of two places, with the same result:
+//  - binned (a.bins set, bin of at most kBinCap entries): the tile's bin,
+//    sorted in LDS (bitonic), so the block touches only the ~tens of tracks
+//    that reach it;
+//  - scan: the block tests every track of its step against the tile and
+//    compacts the hits in arrival order (ballot + prefix count), 256 at a
+//    time.
+// It then evaluates each listed track's 16 row and 16 column factors once
+// into LDS, 128 tracks at a time (few barriers, and sixteen independent exp
+// evaluations per thread). A factor whose row or column lies outside the
+// track's box is stored as zero, so the sum needs no box test: a skipped
+// term and an added +0.0 leave the (non-negative) sum bit-identical.
 // Output: dens = sum * gaussian_factor, and per block the scored injected
 // charge sum(dens * no_z) over interior columns inside the scoring disc
 // (solver_numba_parallel.py:250-277). Steps without tracks are skipped.
@@ -785,10 +843,11 @@ __device__ __forceinline__ void track_box(double x, double y, double c, double n
 // exp(-di^2 h2b2) and exp(-dj^2 h2b2).
 template <bool DERIV>
 __device__ __forceinline__ void gather_body(const GatherArgs &a, const int tile, const int ystep) {
-    __shared__ double s_x[256], s_y[256];  // centres of this chunk's tracks that reach the tile
+    __shared__ double s_x[256], s_y[256];  // centres of listed tracks
     __shared__ int s_wcnt[4];
     __shared__ double s_g[kGB][2 * kGT];
     __shared__ double s_d2[DERIV ? kGB : 1][2 * kGT];  // squared offsets (DERIV)
+    __shared__ int s_list[DERIV ? 1 : kBinCap];        // the tile's bin, sorted
     const int n_tiles = a.ntj * a.ntj;
     const int64_t step = a.first_step + ystep;
     const int64_t t_begin = a.offsets[step];
@@ -804,51 +863,110 @@ __device__ __forceinline__ void gather_body(const GatherArgs &a, const int tile,
     const int i = ti0 + ty, j = tj0 + tx;
     const double nx = (double)a.no_xy;
     double acc = 0.0, acc2 = 0.0;
-    for (int64_t base = 0; base < m; base += 256) {
-        const int64_t t = base + tid;
-        bool hit = false;
-        const double tx_c = xs[min(t, m - 1)], ty_c = ys[min(t, m - 1)];
-        if (t < m) {
+    // factors of the nq listed centres at s_x/s_y[at ...] into s_g, then the
+    // column sums; two barriers
+    auto add_listed = [&](int at, int nq) {
+        for (int e = tid; e < nq * 2 * kGT; e += 256) {
+            const int q = e / (2 * kGT), f = e % (2 * kGT);
+            const double x = s_x[at + q], y = s_y[at + q];
             int li, hi, lj, hj;
-            track_box(tx_c, ty_c, a.cutoff, nx, li, hi, lj, hj);
-            hit = li < ti0 + kGT && hi > ti0 && lj < tj0 + kGT && hj > tj0;
-        }
-        const unsigned long long bal = __ballot(hit);
-        const int before = __popcll(bal & ((1ull << lane) - 1ull));
-        if (lane == 0) s_wcnt[wave] = __popcll(bal);
-        __syncthreads();
-        int off = 0, nlist = 0;
-        for (int w = 0; w < 4; ++w) {
-            if (w < wave) off += s_wcnt[w];
-            nlist += s_wcnt[w];
-        }
-        // the list keeps the tracks' arrival order, so every voxel still adds
-        // them in ascending order
-        if (hit) {
-            s_x[off + before] = tx_c;
-            s_y[off + before] = ty_c;
+            track_box(x, y, a.cutoff, nx, li, hi, lj, hj);
+            const int cell = f < kGT ? ti0 + f : tj0 + f - kGT;
+            const bool in_box = f < kGT ? (cell >= li && cell < hi) : (cell >= lj && cell < hj);
+            const double d = f < kGT ? (double)cell - x : (double)cell - y;
+            s_g[q][f] = in_box ? exp(-(d * d) * a.h2b2) : 0.0;
+            if (DERIV) s_d2[q][f] = d * d;
         }
         __syncthreads();
-        for (int sb = 0; sb < nlist; sb += kGB) {
-            // 128 tracks x 32 factors: sixteen per thread
-            const int nq = min(kGB, nlist - sb);
-            for (int e = tid; e < nq * 2 * kGT; e += 256) {
-                const int q = e / (2 * kGT), f = e % (2 * kGT);
-                const double x = s_x[sb + q], y = s_y[sb + q];
+        for (int q2 = 0; q2 < nq; ++q2) acc += s_g[q2][ty] * s_g[q2][kGT + tx];
+        if (DERIV)
+            for (int q2 = 0; q2 < nq; ++q2)
+                acc2 += (s_g[q2][ty] * s_g[q2][kGT + tx]) * (s_d2[q2][ty] + s_d2[q2][kGT + tx]);
+        __syncthreads();
+    };
+    bool binned = false;
+    if constexpr (!DERIV) {
+        if (a.bins.list) {
+            const int cell = ystep * n_tiles + tile;
+            const int nb = a.bins.cnt[cell];
+            if (nb <= kBinCap) {
+                binned = true;
+                const int base = bin_step_base(a.bins.tot, ystep) + a.bins.off[cell];
+                // sort ascending: the bin was filled in no particular order
+                if (nb <= 256) {
+                    // one entry per thread, placed at its rank (the entries are
+                    // distinct track indices): two barriers
+                    const int v = tid < nb ? a.bins.list[base + tid] : 0;
+                    if (tid < nb) s_list[kBinCap - 256 + tid] = v;
+                    __syncthreads();
+                    if (tid < nb) {
+                        int rank = 0;
+                        for (int u = 0; u < nb; ++u) rank += s_list[kBinCap - 256 + u] < v;
+                        s_list[rank] = v;
+                    }
+                    __syncthreads();
+                } else {
+                    int p2 = 2;
+                    while (p2 < nb) p2 <<= 1;
+                    for (int e = tid; e < p2; e += 256) s_list[e] = e < nb ? a.bins.list[base + e] : INT_MAX;
+                    __syncthreads();
+                    // bitonic
+                    for (int k = 2; k <= p2; k <<= 1) {
+                        for (int h = k >> 1; h > 0; h >>= 1) {
+                            for (int e = tid; e < p2; e += 256) {
+                                const int o = e ^ h;
+                                if (o > e) {
+                                    const int u = s_list[e], w = s_list[o];
+                                    if ((u > w) == ((e & k) == 0)) {
+                                        s_list[e] = w;
+                                        s_list[o] = u;
+                                    }
+                                }
+                            }
+                            __syncthreads();
+                        }
+                    }
+                }
+                for (int sb = 0; sb < nb; sb += kGB) {
+                    const int nq = min(kGB, nb - sb);
+                    if (tid < nq) {
+                        const int t = s_list[sb + tid];
+                        s_x[tid] = xs[t];
+                        s_y[tid] = ys[t];
+                    }
+                    __syncthreads();
+                    add_listed(0, nq);
+                }
+            }
+        }
+    }
+    if (!binned) {
+        for (int64_t base = 0; base < m; base += 256) {
+            const int64_t t = base + tid;
+            bool hit = false;
+            const double tx_c = xs[min(t, m - 1)], ty_c = ys[min(t, m - 1)];
+            if (t < m) {
                 int li, hi, lj, hj;
-                track_box(x, y, a.cutoff, nx, li, hi, lj, hj);
-                const int cell = f < kGT ? ti0 + f : tj0 + f - kGT;
-                const bool in_box = f < kGT ? (cell >= li && cell < hi) : (cell >= lj && cell < hj);
-                const double d = f < kGT ? (double)cell - x : (double)cell - y;
-                s_g[q][f] = in_box ? exp(-(d * d) * a.h2b2) : 0.0;
-                if (DERIV) s_d2[q][f] = d * d;
+                track_box(tx_c, ty_c, a.cutoff, nx, li, hi, lj, hj);
+                hit = li < ti0 + kGT && hi > ti0 && lj < tj0 + kGT && hj > tj0;
+            }
+            const unsigned long long bal = __ballot(hit);
+            const int before = __popcll(bal & ((1ull << lane) - 1ull));
+            if (lane == 0) s_wcnt[wave] = __popcll(bal);
+            __syncthreads();
+            int off = 0, nlist = 0;
+            for (int w = 0; w < 4; ++w) {
+                if (w < wave) off += s_wcnt[w];
+                nlist += s_wcnt[w];
+            }
+            // the list keeps the tracks' arrival order, so every voxel still adds
+            // them in ascending order
+            if (hit) {
+                s_x[off + before] = tx_c;
+                s_y[off + before] = ty_c;
             }
             __syncthreads();
-            for (int q2 = 0; q2 < nq; ++q2) acc += s_g[q2][ty] * s_g[q2][kGT + tx];
-            if (DERIV)
-                for (int q2 = 0; q2 < nq; ++q2)
-                    acc2 += (s_g[q2][ty] * s_g[q2][kGT + tx]) * (s_d2[q2][ty] + s_d2[q2][kGT + tx]);
-            __syncthreads();
+            for (int sb = 0; sb < nlist; sb += kGB) add_listed(sb, min(kGB, nlist - sb));
         }
     }
     double inj = 0.0, z1 = 0.0, z2 = 0.0;
@@ -884,6 +1002,88 @@ __device__ __forceinline__ void gather_body(const GatherArgs &a, const int tile,
 }
 
 __global__ void __launch_bounds__(256) gather_kernel(GatherArgs a) { gather_body<false>(a, blockIdx.x, blockIdx.y); }
+
+// The three passes that bin one launch's tracks by gather tile (BinView).
+struct BinArgs {
+    const double *xs, *ys;
+    const int64_t *offsets;  // device step offsets
+    int64_t first_step;      // step of blockIdx.y == 0
+    int *cnt, *off, *cur, *tot, *list;
+    int64_t n_cnt;           // counters bin_clear_kernel zeroes (steps x tiles)
+    int no_xy, ntj;
+    double cutoff;
+};
+
+// Pass 0: zero the counters of the launch's steps (a kernel rather than a
+// memset, so the pass can be replayed in a graph).
+__global__ void __launch_bounds__(256) bin_clear_kernel(BinArgs a) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < a.n_cnt; e += (int64_t)gridDim.x * blockDim.x)
+        a.cnt[e] = 0;
+}
+
+// Pass 1, grid (any, steps): count the tracks meeting each tile. cnt starts
+// at zero (the host clears it, with a kernel, before the pass).
+__global__ void __launch_bounds__(256) bin_count_kernel(BinArgs a) {
+    const int y = blockIdx.y, nt = a.ntj * a.ntj;
+    const int64_t t0 = a.offsets[a.first_step + y], m = a.offsets[a.first_step + y + 1] - t0;
+    const double nx = (double)a.no_xy;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < m; t += (int64_t)gridDim.x * blockDim.x) {
+        int ti0, ti1, tj0, tj1;
+        if (!track_tiles(a.xs[t0 + t], a.ys[t0 + t], a.cutoff, nx, ti0, ti1, tj0, tj1)) continue;
+        for (int ti = ti0; ti <= ti1; ++ti)
+            for (int tj = tj0; tj <= tj1; ++tj) atomicAdd(&a.cnt[(int64_t)y * nt + ti * a.ntj + tj], 1);
+    }
+}
+
+// Pass 2, one block of 256 per step: exclusive scan of the step's counts into
+// off, the step's total into tot, the fill cursors cleared.
+__global__ void __launch_bounds__(256) bin_scan_kernel(BinArgs a) {
+    __shared__ int s_wsum[4];
+    const int y = blockIdx.x, nt = a.ntj * a.ntj;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t row = (int64_t)y * nt;
+    int carry = 0;
+    for (int b = 0; b < nt; b += 256) {
+        const int e = b + threadIdx.x;
+        const int v = e < nt ? a.cnt[row + e] : 0;
+        int incl = v;
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+            const int u = __shfl_up(incl, o, kWave);
+            if (lane >= o) incl += u;
+        }
+        if (lane == kWave - 1) s_wsum[wave] = incl;
+        __syncthreads();
+        int before = carry;
+        for (int w = 0; w < wave; ++w) before += s_wsum[w];
+        if (e < nt) {
+            a.off[row + e] = before + incl - v;
+            a.cur[row + e] = 0;
+        }
+        carry += s_wsum[0] + s_wsum[1] + s_wsum[2] + s_wsum[3];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) a.tot[y] = carry;
+}
+
+// Pass 3, grid as pass 1: each track's index (within its step) into every
+// bin it meets, at a slot taken from the bin's cursor.
+__global__ void __launch_bounds__(256) bin_fill_kernel(BinArgs a) {
+    const int y = blockIdx.y, nt = a.ntj * a.ntj;
+    const int base = bin_step_base(a.tot, y);
+    const int64_t t0 = a.offsets[a.first_step + y], m = a.offsets[a.first_step + y + 1] - t0;
+    const double nx = (double)a.no_xy;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < m; t += (int64_t)gridDim.x * blockDim.x) {
+        int ti0, ti1, tj0, tj1;
+        if (!track_tiles(a.xs[t0 + t], a.ys[t0 + t], a.cutoff, nx, ti0, ti1, tj0, tj1)) continue;
+        for (int ti = ti0; ti <= ti1; ++ti)
+            for (int tj = tj0; tj <= tj1; ++tj) {
+                const int64_t cell = (int64_t)y * nt + ti * a.ntj + tj;
+                const int slot = atomicAdd(&a.cur[cell], 1);
+                a.list[base + a.off[cell] + slot] = (int)t;
+            }
+    }
+}
 
 // The gathers of many configurations in one launch: block g belongs to
 // configuration c with offs[c] <= g < offs[c+1] and runs tile

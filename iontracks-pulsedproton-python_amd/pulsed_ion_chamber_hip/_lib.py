@@ -25,6 +25,11 @@ PRECISIONS = {"fp64": PIC_FP64, "fp32": PIC_FP32}
 PIC_WALL_ABSORBING = 0
 PIC_WALL_REFLECTING = 1
 
+PIC_GATHER_AUTO = 0
+PIC_GATHER_SCAN = 1
+PIC_GATHER_BINNED = 2
+GATHER_MODES = {"auto": PIC_GATHER_AUTO, "scan": PIC_GATHER_SCAN, "binned": PIC_GATHER_BINNED}
+
 # every symbol include/pic_hip.h declares
 EXPORTED = (
     "pic_last_error",
@@ -43,6 +48,8 @@ EXPORTED = (
     "pic_write_fields",
     "pic_reset",
     "pic_synchronize",
+    "pic_set_gather_mode",
+    "pic_get_gather_mode",
     "pic_set_tiling",
     "pic_get_tiling",
     "pic_enable_timing",
@@ -133,6 +140,8 @@ def open_library():
         "pic_write_fields": [_VP, _DP, _DP],
         "pic_reset": [_VP],
         "pic_synchronize": [_VP],
+        "pic_set_gather_mode": [_VP, ctypes.c_int],
+        "pic_get_gather_mode": [_VP, ctypes.POINTER(ctypes.c_int)],
         "pic_set_tiling": [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int],
         "pic_get_tiling": [_VP] + [ctypes.POINTER(ctypes.c_int)] * 3,
         "pic_enable_timing": [_VP, ctypes.c_int],

@@ -145,6 +145,20 @@ class HipSimulation:
     def synchronize(self) -> None:
         _lib.check(self.L.pic_synchronize(self.handle), "pic_synchronize")
 
+    def set_gather_mode(self, mode: str = "auto") -> None:
+        """How deposition finds the tracks reaching each tile: "auto", "scan"
+        (every tile tests every track) or "binned" (tracks binned by tile on
+        the device first). Same results bit for bit."""
+        if mode not in _lib.GATHER_MODES:
+            raise ValueError(f'gather mode must be "auto", "scan" or "binned", got {mode!r}.')
+        _lib.check(self.L.pic_set_gather_mode(self.handle, _lib.GATHER_MODES[mode]), "pic_set_gather_mode")
+
+    def gather_binned(self) -> bool:
+        """Whether the uploaded tracks are binned by tile."""
+        v = ctypes.c_int()
+        _lib.check(self.L.pic_get_gather_mode(self.handle, ctypes.byref(v)), "pic_get_gather_mode")
+        return bool(v.value)
+
     def set_tiling(self, rows_per_tile: int = 0, planes_per_chunk: int = 0, values_per_lane: int = 0) -> None:
         _lib.check(self.L.pic_set_tiling(self.handle, int(rows_per_tile), int(planes_per_chunk), int(values_per_lane)),
                    "pic_set_tiling")
