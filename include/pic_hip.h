@@ -168,6 +168,10 @@ int pic_synchronize(pic_sim *sim);
  * rows per tile 1 or 2 only); 0 = automatic. Takes effect at the next launch. */
 int pic_set_tiling(pic_sim *sim, int rows_per_tile, int planes_per_chunk, int values_per_lane);
 
+/* Consecutive j tiles per sweep block (0 = automatic, at most 8 waves per
+ * block): neighbouring tiles then run on one CU side by side. */
+int pic_set_tile_group(pic_sim *sim, int tiles_per_block);
+
 /* The tiling the next sweep launch uses: the rule's choice for every knob
  * pic_set_tiling left at 0 (replaces reading back the threads per block the
  * reference's set_threads_per_block stores, solver_cuda.py:121-132). */

@@ -62,8 +62,9 @@ int fail(int code, const char *fmt, ...) {
 int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 struct SweepShape {
-    int tj = 0, chunk = 0, n_windows = 0, wpb = 0, n_kg = 0, n_jt = 0, n_ic = 0, v = 1, win_step = 0;
-    int64_t blocks() const { return (int64_t)n_kg * n_jt * n_ic; }
+    int tj = 0, chunk = 0, n_windows = 0, wpb = 0, n_kg = 0, n_jt = 0, n_ic = 0, v = 1, win_step = 0, tpb = 1;
+    int64_t blocks() const { return (int64_t)n_kg * ((n_jt + tpb - 1) / tpb) * n_ic; }
+    int threads() const { return 64 * wpb * tpb; }
 };
 
 // One kernel launch held as data: the kernel, its shape and a copy of its
@@ -118,12 +119,13 @@ struct pic_sim {
         double *out4 = nullptr;
     } pending;
     double *scratch4 = nullptr;  // 8 doubles for the kernel-level calls
+    double *zero_dens = nullptr; // [no_xy^2] zeros: the density a step without tracks adds
     double *per_step = nullptr;
     int64_t n_steps = 0;
     double *xs = nullptr, *ys = nullptr;
     int64_t n_tracks = 0;
     std::vector<int64_t> offsets;
-    int force_tj = 0, force_chunk = 0, force_v = 0;
+    int force_tj = 0, force_chunk = 0, force_v = 0, force_tpb = 0;
     int cus = 256;                 // compute units of the device (queried once)
     bool shape_valid = false;      // `shape` matches the tiling knobs
     SweepShape shape;
@@ -323,14 +325,22 @@ SweepShape compute_sweep_shape(const pic_sim *s) {
             }
         }
     }
+    // Four values per lane: two tiles per block marching in lockstep, 32
+    // planes (1066^2 x 410: 1.58 ms per step, against 1.59 for 16 or 64
+    // planes, 1.63 and 1.60 for four tiles of 32 or 64, 1.69 for one tile of
+    // 16 free-running). Smaller grids, with tiles of four rows and two values,
+    // were measured slower in lockstep (208^2 x 406: 75 us alone, 82-97 us in
+    // pairs or fours) and keep one tile per block.
+    const int tpb = s->force_tpb > 0 ? std::min(s->force_tpb, std::max(1, 8 / sh.wpb)) : (sh.v == 4 ? 2 : 1);
     if (chunk <= 0) {
-        chunk = chunk0;
+        chunk = sh.v == 4 ? 32 : chunk0;
         while (chunk > 4 && ceil_div(rows, tj) * sh.n_windows * ceil_div(planes, chunk) < waves_wanted) chunk /= 2;
     }
     sh.tj = tj;
     sh.n_jt = (int)ceil_div(rows, tj);
     sh.chunk = (int)std::min<int64_t>(chunk, planes);
     sh.n_ic = (int)ceil_div(planes, sh.chunk);
+    sh.tpb = tpb;
     return sh;
 }
 
@@ -636,6 +646,7 @@ int make_sweep_args(pic_sim *s, const SweepShape &sh, int64_t step, bool dep, co
     a.n_windows = sh.n_windows;
     a.win_step = sh.win_step;
     a.wpb = sh.wpb;
+    a.tpb = sh.tpb;
     a.n_kg = sh.n_kg;
     a.n_jt = sh.n_jt;
     a.n_ic = sh.n_ic;
@@ -662,8 +673,16 @@ int enqueue_sweep(pic_sim *s, int64_t step, bool dep, const double *dens, const 
     int rc = make_sweep_args(s, sh, step, dep, dens, dep_part, n_dep, out4, a);
     if (rc) return rc;
     // one extra block folds the previous step when one is pending
-    const dim3 grid((unsigned)(sh.blocks() + (a.prev_out4 ? 1 : 0))), block((unsigned)(64 * sh.wpb));
-    const void *f = checked_sweep_fn(s, sh, dep);
+    const dim3 grid((unsigned)(sh.blocks() + (a.prev_out4 ? 1 : 0))), block((unsigned)sh.threads());
+    // A step without tracks runs the depositing kernel too, adding a density
+    // of zeros (x + 0 == x, so the fields are the same bit for bit). Measured
+    // on the 5 um full electrode: the non-depositing variant, whose march
+    // issues its loads with less work between them, read 1.43x the
+    // algorithmic bytes against 1.17x (the neighbouring tiles' halo rows hit
+    // the L2 less often) and took 1.79 against 1.68 ms per step.
+    a.dens = s->zero_dens;
+    if (dep) a.dens = dens;
+    const void *f = checked_sweep_fn(s, sh, true);
     if (!f) return PIC_E_INVALID;
     return emit(s, KLaunch::make(f, grid, block, 0, a), s->stream);
 }
@@ -1107,7 +1126,7 @@ int adjoint_recompute_sweep(pic_adjoint *ad, int64_t t, int64_t src) {
     a.pos_out = in + 2 * s->n_vox;
     a.neg_out = in + 3 * s->n_vox;
     a.prev_out4 = nullptr;
-    const dim3 grid((unsigned)sh.blocks()), block((unsigned)(64 * sh.wpb));
+    const dim3 grid((unsigned)sh.blocks()), block((unsigned)sh.threads());
     const void *f = checked_sweep_fn(s, sh, dep);
     if (!f) return PIC_E_INVALID;
     return launch_now(KLaunch::make(f, grid, block, 0, a), s->stream);
@@ -1221,7 +1240,7 @@ int pic_mem_required(const pic_params *p, int precision, size_t *bytes) {
     const size_t xy = (size_t)p->no_xy * p->no_xy;
     const size_t gt = (size_t)ceil_div(p->no_xy, pic::kGT);
     const size_t b = (size_t)dens_slots(batch_steps(p));
-    *bytes = 4 * vox * elem + b * (xy * 8 + gt * gt * 8) + 2 * partial_cap_bytes(p) + 256;
+    *bytes = 4 * vox * elem + b * (xy * 8 + gt * gt * 8) + xy * 8 + 2 * partial_cap_bytes(p) + 256;
     return PIC_OK;
 }
 
@@ -1270,9 +1289,10 @@ int pic_create(const pic_params *p, int precision, pic_sim **out) {
     if (hipMalloc(&s->dens, slots * xy * 8) != hipSuccess ||
         hipMalloc(&s->sweep_part, 2 * s->sweep_part_cap * 3 * 8) != hipSuccess ||
         hipMalloc(&s->dep_part, slots * gather_blocks(s) * 8) != hipSuccess ||
-        hipMalloc(&s->scratch4, 8 * 8) != hipSuccess)
+        hipMalloc(&s->scratch4, 8 * 8) != hipSuccess || hipMalloc(&s->zero_dens, xy * 8) != hipSuccess)
         return cleanup(fail(PIC_E_NOMEM, "hipMalloc of scratch failed"));
     if (hipMemsetAsync(s->dens, 0, slots * xy * 8, s->stream) != hipSuccess ||
+        hipMemsetAsync(s->zero_dens, 0, xy * 8, s->stream) != hipSuccess ||
         hipMemsetAsync(s->scratch4, 0, 64, s->stream) != hipSuccess)
         return cleanup(fail(PIC_E_HIP, "hipMemsetAsync failed"));
     if (hipStreamSynchronize(s->stream) != hipSuccess) return cleanup(fail(PIC_E_HIP, "initialisation failed"));
@@ -1285,7 +1305,7 @@ int pic_destroy(pic_sim *s) {
     if (s->stream) hipStreamSynchronize(s->stream);
     if (s->gstream) hipStreamSynchronize(s->gstream);
     void *bufs[] = {s->cur_p, s->cur_n, s->nxt_p, s->nxt_n, s->dens, s->sweep_part, s->dep_part,
-                    s->scratch4, s->per_step, s->xs, s->ys, s->offsets_d};
+                    s->scratch4, s->zero_dens, s->per_step, s->xs, s->ys, s->offsets_d};
     for (void *b : bufs)
         if (b) hipFree(b);
     free_bins(s);
@@ -1711,6 +1731,15 @@ int pic_set_tiling(pic_sim *s, int rows_per_tile, int planes_per_chunk, int valu
         s->shape_valid = false;
     }
     return rc;
+}
+
+int pic_set_tile_group(pic_sim *s, int tiles_per_block) {
+    if (!s) return fail(PIC_E_INVALID, "sim is NULL");
+    if (tiles_per_block < 0 || tiles_per_block > 8) return fail(PIC_E_INVALID, "tiles_per_block must be 0..8");
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->force_tpb = tiles_per_block;
+    s->shape_valid = false;
+    return ensure_partials(s, sweep_shape(s).blocks());
 }
 
 int pic_get_tiling(pic_sim *s, int *rows_per_tile, int *planes_per_chunk, int *values_per_lane) {
