@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <climits>
 #include <cstdint>
 #include <cstdarg>
@@ -416,23 +417,37 @@ int launch_now(const KLaunch &l, hipStream_t st) {
 // every node) for each use, so a run's launches cost one graph launch per
 // kChain kernels instead of one launch each, and the device starts each
 // kernel as the previous one ends. Instances are pooled per device; one is
-// re-parameterised only after the event recorded behind its last launch has
-// completed, so an update never touches a launch still in flight. The pool
-// lives until the process ends.
+// re-parameterised only after its previous launch has finished, so an update
+// never touches a launch in flight.
+//
+// Completion is read from memory, not asked of the runtime: a last node
+// (chain_mark_kernel) stores the launch's number to a pinned host word. The
+// HIP runtime refuses calls that are unsafe during stream capture -- event
+// queries and synchronisations among them -- from every thread while any
+// stream of the process is capturing, and its graph machinery may capture
+// while another thread instantiates or launches a graph; an event query here
+// was refused that way ("stream is capturing") when two threads drove chains
+// at once. The pool therefore makes no such call: under g_chain_mu it only
+// reads the marks, sets node parameters and launches graphs.
 constexpr int kChain = 32;
 constexpr int kChainPool = 4;
 
 struct Chain {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    hipGraphNode_t nodes[kChain] = {};
-    hipEvent_t done = nullptr;
+    hipGraphNode_t nodes[kChain + 1] = {};  // kChain kernels, then the mark
+    unsigned long long *mark = nullptr;     // pinned host word the last node writes
+    unsigned long long seq = 0;             // number of the last launch
     uint64_t last_use = 0;
 };
 
 std::mutex g_chain_mu;
 std::vector<std::vector<Chain *>> g_chains;  // [device]
-uint64_t g_chain_uses = 0;
+unsigned long long g_chain_uses = 0;
+
+bool chain_done(const Chain *c) {
+    return __atomic_load_n(c->mark, __ATOMIC_ACQUIRE) >= c->seq;
+}
 
 hipKernelNodeParams node_params(const KLaunch &l, void **argv) {
     hipKernelNodeParams p{};
@@ -446,28 +461,33 @@ hipKernelNodeParams node_params(const KLaunch &l, void **argv) {
     return p;
 }
 
-int chain_create(const KLaunch *ls, Chain **out) {
+KLaunch mark_launch(Chain *c, unsigned long long seq) {
+    return KLaunch::make((const void *)pic::chain_mark_kernel, dim3(1), dim3(64), 2, pic::ChainMark{c->mark, seq});
+}
+
+int chain_create(const KLaunch *ls, unsigned long long seq, Chain **out) {
     Chain *c = new Chain();
     auto cleanup = [&](int code) {
         if (c->exec) hipGraphExecDestroy(c->exec);
         if (c->graph) hipGraphDestroy(c->graph);
-        if (c->done) hipEventDestroy(c->done);
+        if (c->mark) hipHostFree(c->mark);
         delete c;
         return code;
     };
+    if (hipHostMalloc((void **)&c->mark, sizeof(unsigned long long), hipHostMallocCoherent) != hipSuccess)
+        return cleanup(fail(PIC_E_NOMEM, "hipHostMalloc of a chain mark failed"));
+    *c->mark = 0;
     if (hipGraphCreate(&c->graph, 0) != hipSuccess) return cleanup(fail(PIC_E_HIP, "hipGraphCreate failed"));
-    for (int i = 0; i < kChain; ++i) {
+    const KLaunch mark = mark_launch(c, seq);
+    for (int i = 0; i <= kChain; ++i) {
         void *argv[1];
-        const hipKernelNodeParams p = node_params(ls[i], argv);
+        const hipKernelNodeParams p = node_params(i < kChain ? ls[i] : mark, argv);
         if (hipGraphAddKernelNode(&c->nodes[i], c->graph, i ? &c->nodes[i - 1] : nullptr, i ? 1 : 0, &p) != hipSuccess)
             return cleanup(fail(PIC_E_HIP, "hipGraphAddKernelNode failed"));
     }
     if (hipGraphInstantiate(&c->exec, c->graph, nullptr, nullptr, 0) != hipSuccess)
         return cleanup(fail(PIC_E_HIP, "hipGraphInstantiate failed"));
-    // device-scope release: the event only marks the launch as consumed; a
-    // system-scope one would write the L2 back to memory behind every chain
-    if (hipEventCreateWithFlags(&c->done, hipEventDisableTiming | hipEventReleaseToDevice) != hipSuccess)
-        return cleanup(fail(PIC_E_HIP, "hipEventCreate failed"));
+    c->seq = seq;
     *out = c;
     return PIC_OK;
 }
@@ -477,35 +497,40 @@ int chain_launch(const KLaunch *ls, int device, hipStream_t st) {
     std::lock_guard<std::mutex> lock(g_chain_mu);
     if ((int)g_chains.size() <= device) g_chains.resize(device + 1);
     std::vector<Chain *> &pool = g_chains[device];
+    const unsigned long long seq = ++g_chain_uses;
     Chain *c = nullptr;
     bool fresh = false;
-    for (Chain *q : pool) {
-        const hipError_t e = hipEventQuery(q->done);
-        if (e == hipSuccess) {
+    for (Chain *q : pool)
+        if (chain_done(q)) {
             c = q;
             break;
         }
-        if (e != hipErrorNotReady) return fail(PIC_E_HIP, "hipEventQuery failed: %s", hipGetErrorString(e));
-    }
     if (!c && (int)pool.size() < kChainPool) {
-        int rc = chain_create(ls, &c);
+        int rc = chain_create(ls, seq, &c);
         if (rc) return rc;
         pool.push_back(c);
         fresh = true;
     }
-    if (!c) {  // all in flight: wait for the least recently launched
+    if (!c) {  // all in flight: wait for the least recently launched to finish
         c = *std::min_element(pool.begin(), pool.end(), [](Chain *x, Chain *y) { return x->last_use < y->last_use; });
-        HIP_TRY(hipEventSynchronize(c->done));
+        const auto t0 = std::chrono::steady_clock::now();
+        while (!chain_done(c)) {
+            if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(120))
+                return fail(PIC_E_HIP, "a graph chain launched earlier did not finish within 120 s");
+            std::this_thread::yield();
+        }
     }
-    if (!fresh)
-        for (int i = 0; i < kChain; ++i) {
+    if (!fresh) {
+        for (int i = 0; i <= kChain; ++i) {
             void *argv[1];
-            hipKernelNodeParams p = node_params(ls[i], argv);
+            const KLaunch mark = mark_launch(c, seq);
+            hipKernelNodeParams p = node_params(i < kChain ? ls[i] : mark, argv);
             HIP_TRY(hipGraphExecKernelNodeSetParams(c->exec, c->nodes[i], &p));
         }
+        c->seq = seq;
+    }
     HIP_TRY(hipGraphLaunch(c->exec, st));
-    HIP_TRY(hipEventRecord(c->done, st));
-    c->last_use = ++g_chain_uses;
+    c->last_use = seq;
     return PIC_OK;
 }
 
@@ -1908,10 +1933,8 @@ int pic_batch_run(pic_batch *b, int64_t first, int64_t count) {
     if (first < 0 || count < 0 || first + count > longest)
         return fail(PIC_E_INVALID, "steps [%lld, %lld) outside the longest uploaded run (%lld)", (long long)first,
                     (long long)(first + count), (long long)longest);
-    // lanes drive launches from two host threads: not while timing, and not
-    // with graph replay -- graph chains driven from two threads at once have
-    // had a queried event refused by the runtime ("stream is capturing")
-    if (!b->is_lane && !b->timing && !b->graphs) {
+    // lanes drive launches from two host threads: not while timing
+    if (!b->is_lane && !b->timing) {
         std::vector<int> part;
         if (lane_split(b, part)) return run_lanes(b, part, first, count);
     }

@@ -1175,6 +1175,19 @@ __global__ void __launch_bounds__(256) ring_mirror_kernel(S *pos, S *neg, int no
     }
 }
 
+// The last node of every graph chain: stores the chain's launch number to
+// pinned host memory once every node before it has run, so the host can tell
+// a finished chain from one in flight by reading memory, with no runtime call.
+// The store is relaxed: the host needs to know that the nodes have finished,
+// not to see their data, and a release here would write back the whole L2.
+struct ChainMark {
+    unsigned long long *flag;
+    unsigned long long seq;
+};
+__global__ void __launch_bounds__(64) chain_mark_kernel(ChainMark m) {
+    if (threadIdx.x == 0) __hip_atomic_store(m.flag, m.seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 template <typename S>
 __global__ void __launch_bounds__(256) fill_kernel(S *p, int64_t n) {
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
