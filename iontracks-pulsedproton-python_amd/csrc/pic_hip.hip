@@ -309,22 +309,31 @@ SweepShape compute_sweep_shape(const pic_sim *s) {
     // Four values per lane: the 2-row tile (175-189 registers, two waves per
     // SIMD), with chunks shortened first until there are ~16 waves per CU --
     // two rounds of the 8 a CU holds (208^2 x 406: 8 planes, 78.5 us, against
-    // 84.6 for 16; 1066^2 x 410: 16 planes, 1.68 ms, against 1.73 for 8).
-    const int64_t waves_wanted = (int64_t)cus * (sh.v == 4 ? 16 : 8);
+    // 84.6 for 16). Two values per lane aim for 16 waves per CU too: on
+    // 208^2 x 406, 4-row tiles of 8 planes (5408 waves) against 16 (2704),
+    // measured on three boxes: 72.1 against 74.2, 71.2 against 73.2 and 75.2
+    // against 78.4 us.
+    const int64_t waves_wanted = (int64_t)cus * (sh.v >= 2 ? 16 : 8);
     int tj = s->force_tj;
     int chunk = s->force_chunk;
     const int max_tj = (sh.v == 1 && s->prec == PIC_FP32) ? 8 : sh.v == 4 ? 2 : 4;
     const int chunk0 = 16;
     if (tj <= 0 && sh.v == 4) tj = max_tj;
     if (tj <= 0) {
+        // the tallest tile that reaches the wave count, with two values per
+        // lane at chunks of 16 or, failing that, 8 planes before a shorter
+        // tile is tried (208^2 x 406: 4-row tiles of 8 planes, not 2-row
+        // tiles of 16, which took 78.9 against 72-75 us)
         const int c0 = chunk > 0 ? chunk : chunk0;
+        const int c_min = chunk > 0 || sh.v != 2 ? c0 : 8;
         tj = 1;
-        for (int c = max_tj; c >= 1; c /= 2) {
-            if (ceil_div(rows, c) * sh.n_windows * ceil_div(planes, c0) >= waves_wanted) {
-                tj = c;
-                break;
-            }
-        }
+        bool found = false;
+        for (int c = max_tj; c >= 1 && !found; c /= 2)
+            for (int ch = c0; ch >= c_min && !found; ch /= 2)
+                if (ceil_div(rows, c) * sh.n_windows * ceil_div(planes, ch) >= waves_wanted) {
+                    tj = c;
+                    found = true;
+                }
     }
     // Four values per lane: two tiles per block marching in lockstep, 32
     // planes (1066^2 x 410: 1.58 ms per step, against 1.59 for 16 or 64
