@@ -312,7 +312,9 @@ def curve_bench(precision, cpu_budget_s):
 
 def adjoint_bench(fd: bool = True, pulses: int = 3):
     """dk_s/dtheta on the 10 um, r 0.02 cm grid over 3 pulses (FP64 fields,
-    checkpoints every 64 steps): forward and reverse time, device memory, and
+    segments of 64 steps, their starts kept in two levels): forward and
+    reverse time (the reverse includes one more forward pass that recomputes
+    the segment starts), device memory, and
     the gradient against Richardson-extrapolated FP64 central differences
     (relative steps 1e-3 and 2e-3, dt and step count frozen) on all six
     parameters."""
@@ -342,6 +344,9 @@ def adjoint_bench(fd: bool = True, pulses: int = 3):
         "gradient_wall_s": wall,
         "plain_forward_wall_s": plain,
         "checkpoint_every": info["checkpoint_every"],
+        "segments": info["segments"],
+        "segments_per_outer_checkpoint": info["segments_per_outer"],
+        "outer_checkpoints": info["outer_checkpoints"],
         "adjoint_device_bytes": info["device_bytes"],
         "plain_state_bytes": 4 * cfg.no_xy**2 * cfg.no_z_with_buffer * 8,
     }
@@ -548,7 +553,8 @@ def main():
             "dtype": "f64",
             "data": "synthetic",
             "config": {"workload": f"adjoint_10um_r0.02cm_{args.adjoint_pulses}pulses", "grid": adj["grid"], "fields": "fp64",
-                       "checkpoint_every": adj["checkpoint_every"]},
+                       "checkpoint_every": adj["checkpoint_every"],
+                       "segments_per_outer_checkpoint": adj["segments_per_outer_checkpoint"]},
             "adjoint": adj,
         }
         print(json.dumps(out))

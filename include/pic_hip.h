@@ -248,12 +248,25 @@ int pic_batch_set_graphs(pic_batch *batch, int enable);
  *   d_inj = d(sum injected) / d(gaussian_factor, h2_over_b2).
  * k_s = sum_inj / (sum_inj - sum_rec) and its derivatives follow on the host. */
 int pic_adjoint_create(pic_sim *sim, int64_t checkpoint_every, pic_adjoint **out);
+/* As pic_adjoint_create, with the checkpoints in two levels: the forward pass
+ * keeps the state at the start of every segments_per_outer-th segment only,
+ * and the reverse pass reruns one such span of segments forward (keeping each
+ * segment's start) before differentiating through it -- one more forward
+ * pass, for ceil(segments / M) + M states held instead of one per segment.
+ * 0 picks M = ceil(sqrt(segments)) when there are more than 64 segments and
+ * 1 (one level, as pic_adjoint_create) otherwise. The derivatives do not
+ * depend on the choice: the recomputed states are the forward's, bit for bit. */
+int pic_adjoint_create_levels(pic_sim *sim, int64_t checkpoint_every, int64_t segments_per_outer,
+                              pic_adjoint **out);
 int pic_adjoint_destroy(pic_adjoint *adj);
 int pic_adjoint_run(pic_adjoint *adj, double sums[2], double d_rec[11], double d_inj[2]);
 /* Segment length in use, device bytes held, and the last run's forward and
  * reverse times (HIP events on the handle's stream). */
 int pic_adjoint_get_info(pic_adjoint *adj, int64_t *checkpoint_every, size_t *device_bytes, double *forward_ms,
                          double *backward_ms);
+/* Segments per outer checkpoint (M), outer checkpoints held, and segments. */
+int pic_adjoint_get_levels(pic_adjoint *adj, int64_t *segments_per_outer, int64_t *outer_checkpoints,
+                           int64_t *segments);
 
 #ifdef __cplusplus
 }

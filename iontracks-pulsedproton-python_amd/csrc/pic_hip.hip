@@ -204,8 +204,11 @@ struct pic_batch {
 struct pic_adjoint {
     pic_sim *s = nullptr;
     int64_t K = 0, nseg = 0, T = 0;
+    int64_t M = 1;            // segments per outer checkpoint (1: every segment's state is kept)
+    int64_t n_outer = 0;      // ceil(nseg / M)
     int chunk = 2;            // i planes per adjoint block
-    double *ckpt = nullptr;   // [nseg][2][n_vox]
+    double *ckpt = nullptr;   // [n_outer][2][n_vox]: the state at the start of every M-th segment
+    double *inner = nullptr;  // [M][2][n_vox] (M > 1): one outer span's segment states, recomputed
     double *store = nullptr;  // [K + 1][2][n_vox]
     double *mu = nullptr;     // [2 buffers][2 species][n_vox]
     double *dgf = nullptr, *dh = nullptr;  // [K][no_xy^2]
@@ -2116,6 +2119,10 @@ int pic_batch_synchronize(pic_batch *b) {
 }
 
 int pic_adjoint_create(pic_sim *s, int64_t checkpoint_every, pic_adjoint **out) {
+    return pic_adjoint_create_levels(s, checkpoint_every, 0, out);
+}
+
+int pic_adjoint_create_levels(pic_sim *s, int64_t checkpoint_every, int64_t segments_per_outer, pic_adjoint **out) {
     if (!out) return fail(PIC_E_INVALID, "out is NULL");
     *out = nullptr;
     if (!s) return fail(PIC_E_INVALID, "sim is NULL");
@@ -2123,6 +2130,7 @@ int pic_adjoint_create(pic_sim *s, int64_t checkpoint_every, pic_adjoint **out) 
     if (s->p.no_xy < 4) return fail(PIC_E_INVALID, "the adjoint needs no_xy >= 4");
     if (s->n_steps <= 0) return fail(PIC_E_INVALID, "upload the tracks before creating the adjoint");
     if (checkpoint_every < 1) return fail(PIC_E_INVALID, "checkpoint_every must be positive");
+    if (segments_per_outer < 0) return fail(PIC_E_INVALID, "segments_per_outer must be 0 (automatic) or positive");
     if (s->n_vox * 8 >= INT32_MAX)  // the adjoint kernel addresses each array through a 32-bit buffer offset
         return fail(PIC_E_INVALID, "the adjoint needs arrays below 2 GB (this grid has %lld voxels)", (long long)s->n_vox);
     pic_adjoint *ad = new pic_adjoint();
@@ -2132,15 +2140,25 @@ int pic_adjoint_create(pic_sim *s, int64_t checkpoint_every, pic_adjoint **out) 
     ad->K = std::min<int64_t>(checkpoint_every, s->batch);
     ad->T = s->n_steps;
     ad->nseg = ceil_div(ad->T, ad->K);
+    // Two levels when there are many segments: the forward keeps every M-th
+    // segment's starting state, and the reverse pass recomputes one outer
+    // span of M segments at a time to recover the others -- one more forward
+    // pass in all, for about 2 sqrt(nseg) states held instead of nseg
+    // (3 pulses of 48^2 x 206 at K = 64: 1659 segments, M = 41).
+    if (segments_per_outer == 0)
+        ad->M = ad->nseg > 64 ? (int64_t)std::ceil(std::sqrt((double)ad->nseg)) : 1;
+    else
+        ad->M = std::min(segments_per_outer, ad->nseg);
+    ad->n_outer = ceil_div(ad->nseg, ad->M);
     ad->chunk = adjoint_chunk(s);
     const int64_t xy = s->p.no_xy * s->p.no_xy, tiles = gather_blocks(s), nblk = adjoint_blocks(ad);
     const size_t pair = 2 * (size_t)s->n_vox * 8;
-    const size_t sizes[] = {ad->nseg * pair, (size_t)(ad->K + 1) * pair, 2 * pair, (size_t)ad->K * xy * 8,
+    const size_t sizes[] = {ad->n_outer * pair, (size_t)(ad->K + 1) * pair, 2 * pair, (size_t)ad->K * xy * 8,
                             (size_t)ad->K * xy * 8, (size_t)ad->K * tiles * 2 * 8,
                             (size_t)nblk * 2 * pic::kAdjNV * 8, (size_t)pic::kAdjNV * 8,
-                            (size_t)ad->T * 2 * 8, 64, (size_t)xy * 8};
+                            (size_t)ad->T * 2 * 8, 64, (size_t)xy * 8, ad->M > 1 ? (size_t)ad->M * pair : 0};
     double **ptrs[] = {&ad->ckpt, &ad->store, &ad->mu, &ad->dgf, &ad->dh, &ad->dpart,
-                       &ad->acc, &ad->grad, &ad->dinj_rec, &ad->scratch, &ad->zeros};
+                       &ad->acc, &ad->grad, &ad->dinj_rec, &ad->scratch, &ad->zeros, &ad->inner};
     size_t total = 0;
     for (size_t b : sizes) total += b;
     size_t fr = 0, tot = 0;
@@ -2154,6 +2172,7 @@ int pic_adjoint_create(pic_sim *s, int64_t checkpoint_every, pic_adjoint **out) 
                     total / 1073741824.0, (long long)ad->K, fr / 1073741824.0);
     }
     for (size_t q = 0; q < sizeof sizes / sizeof sizes[0]; ++q) {
+        if (sizes[q] == 0) continue;
         if (hipMalloc(ptrs[q], sizes[q]) != hipSuccess) {
             pic_adjoint_destroy(ad);
             return fail(PIC_E_NOMEM, "hipMalloc for the adjoint failed");
@@ -2177,7 +2196,7 @@ int pic_adjoint_destroy(pic_adjoint *ad) {
     if (!ad) return PIC_OK;
     if (ad->s && ad->s->stream) hipStreamSynchronize(ad->s->stream);
     double *bufs[] = {ad->ckpt, ad->store, ad->mu, ad->dgf, ad->dh, ad->dpart, ad->acc, ad->grad,
-                      ad->dinj_rec, ad->scratch, ad->zeros};
+                      ad->dinj_rec, ad->scratch, ad->zeros, ad->inner};
     for (double *b : bufs)
         if (b) hipFree(b);
     for (hipEvent_t e : ad->ev)
@@ -2193,13 +2212,16 @@ int pic_adjoint_run(pic_adjoint *ad, double sums[2], double d_rec[11], double d_
     const int64_t T = ad->T, K = ad->K, nv = s->n_vox;
     int rc = pic_reset(s);
     if (rc) return rc;
-    // forward, a checkpoint at the start of every segment
-    HIP_TRY(hipEventRecord(ad->ev[0], s->stream));
-    for (int64_t k = 0; k < ad->nseg && rc == PIC_OK; ++k) {
-        double *c = ad->ckpt + k * 2 * nv;
+    // forward, a checkpoint at the start of every M-th segment
+    auto save_state = [&](double *c) -> int {
         HIP_TRY(hipMemcpyAsync(c, s->cur_p, nv * 8, hipMemcpyDeviceToDevice, s->stream));
         HIP_TRY(hipMemcpyAsync(c + nv, s->cur_n, nv * 8, hipMemcpyDeviceToDevice, s->stream));
-        rc = pic_run_steps(s, k * K, std::min(K, T - k * K));
+        return PIC_OK;
+    };
+    HIP_TRY(hipEventRecord(ad->ev[0], s->stream));
+    for (int64_t k = 0; k < ad->nseg && rc == PIC_OK; ++k) {
+        if (k % ad->M == 0) rc = save_state(ad->ckpt + (k / ad->M) * 2 * nv);
+        if (!rc) rc = pic_run_steps(s, k * K, std::min(K, T - k * K));
     }
     if (rc) return rc;
     HIP_TRY(hipEventRecord(ad->ev[1], s->stream));
@@ -2215,7 +2237,25 @@ int pic_adjoint_run(pic_adjoint *ad, double sums[2], double d_rec[11], double d_
     int cur = 0;
     for (int64_t k = ad->nseg - 1; k >= 0 && rc == PIC_OK; --k) {
         const int64_t s0 = k * K, e = std::min(T, s0 + K);
-        HIP_TRY(hipMemcpyAsync(ad->store, ad->ckpt + k * 2 * nv, 2 * nv * 8, hipMemcpyDeviceToDevice, s->stream));
+        const double *start = ad->ckpt + k * 2 * nv;
+        if (ad->M > 1) {
+            // entering an outer span from its end: rerun it forward from its
+            // checkpoint, keeping each segment's starting state (the same
+            // launches as the forward pass, so the same states bit for bit)
+            const int64_t o = k / ad->M, k0 = o * ad->M;
+            if (k == std::min(ad->nseg, k0 + ad->M) - 1) {
+                HIP_TRY(hipMemcpyAsync(s->cur_p, ad->ckpt + o * 2 * nv, nv * 8, hipMemcpyDeviceToDevice, s->stream));
+                HIP_TRY(hipMemcpyAsync(s->cur_n, ad->ckpt + o * 2 * nv + nv, nv * 8, hipMemcpyDeviceToDevice,
+                                       s->stream));
+                for (int64_t q = k0; q <= k && rc == PIC_OK; ++q) {
+                    rc = save_state(ad->inner + (q - k0) * 2 * nv);
+                    if (!rc && q < k) rc = pic_run_steps(s, q * K, std::min(K, T - q * K));
+                }
+                if (rc) break;
+            }
+            start = ad->inner + (k - k0) * 2 * nv;
+        }
+        HIP_TRY(hipMemcpyAsync(ad->store, start, 2 * nv * 8, hipMemcpyDeviceToDevice, s->stream));
         if (s->offsets[e] > s->offsets[s0]) {
             rc = enqueue_gather(s, s->xs, s->ys, s->offsets_d, s0, e - s0, s->stream, s->offsets.data());
             if (!rc) rc = adjoint_deriv_gather(ad, s0, e - s0);
@@ -2266,6 +2306,15 @@ int pic_adjoint_get_info(pic_adjoint *ad, int64_t *checkpoint_every, size_t *dev
     if (device_bytes) *device_bytes = ad->bytes;
     if (forward_ms) *forward_ms = ad->forward_ms;
     if (backward_ms) *backward_ms = ad->backward_ms;
+    return PIC_OK;
+}
+
+int pic_adjoint_get_levels(pic_adjoint *ad, int64_t *segments_per_outer, int64_t *outer_checkpoints,
+                           int64_t *segments) {
+    if (!ad) return fail(PIC_E_INVALID, "adjoint is NULL");
+    if (segments_per_outer) *segments_per_outer = ad->M;
+    if (outer_checkpoints) *outer_checkpoints = ad->n_outer;
+    if (segments) *segments = ad->nseg;
     return PIC_OK;
 }
 

@@ -66,9 +66,11 @@ EXPORTED = (
     "pic_batch_get_timing",
     "pic_batch_set_graphs",
     "pic_adjoint_create",
+    "pic_adjoint_create_levels",
     "pic_adjoint_destroy",
     "pic_adjoint_run",
     "pic_adjoint_get_info",
+    "pic_adjoint_get_levels",
 )
 
 
@@ -162,6 +164,8 @@ def open_library():
         "pic_adjoint_destroy": [_VP],
         "pic_adjoint_run": [_VP, _DP, _DP, _DP],
         "pic_adjoint_get_info": [_VP, _IP, ctypes.POINTER(ctypes.c_size_t), _DP, _DP],
+        "pic_adjoint_create_levels": [_VP, ctypes.c_int64, ctypes.c_int64, ctypes.POINTER(_VP)],
+        "pic_adjoint_get_levels": [_VP, _IP, _IP, _IP],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

@@ -16,9 +16,12 @@ so a run at theta0 is the configuration's own run, and k_s(theta) is smooth.
 
 The library's reverse pass (pic_adjoint_run) returns the derivatives of the
 summed recombined and injected charge with respect to the pic_params fields;
-the chain rule to theta and k_s = I/(I - R) is applied here. Memory: one
-state checkpoint every ``checkpoint_every`` steps plus one segment of
-recomputed states.
+the chain rule to theta and k_s = I/(I - R) is applied here. Memory: the
+state at the start of every ``segments_per_outer``-th segment of
+``checkpoint_every`` steps, the segment starts of one such span (recomputed
+by one extra forward pass over the run) and one segment of recomputed
+states; ``segments_per_outer`` = 0 takes about the square root of the
+segment count on long runs and keeps every segment start on short ones.
 """
 
 import ctypes
@@ -39,7 +42,8 @@ PARAMS = ("alpha", "mu_positive", "mu_negative", "D_positive", "D_negative", "tr
 class KsModel:
     """k_s of one configuration as a function of theta (see the module)."""
 
-    def __init__(self, config, rng: Optional[np.random.Generator] = None, checkpoint_every: int = 64):
+    def __init__(self, config, rng: Optional[np.random.Generator] = None, checkpoint_every: int = 64,
+                 segments_per_outer: int = 0):
         self.config = config
         self.schedule, self.xs, self.ys = draw_tracks(config, rng)
         self.theta0 = np.array([RECOMBINATION_ALPHA_CM3_S, config.mu_positive, config.mu_negative,
@@ -47,6 +51,7 @@ class KsModel:
         b0 = config.track_radius_cm
         self.line_charge = config.Gaussian_factor * math.pi * b0 * b0  # N0
         self.checkpoint_every = int(checkpoint_every)
+        self.segments_per_outer = int(segments_per_outer)
         self.last_info = {}
 
     # --- theta -> the C struct ------------------------------------------
@@ -87,7 +92,8 @@ class KsModel:
         with HipSimulation(self.config, "fp64", params=self.params(theta)) as sim:
             sim.upload_tracks(self.schedule, self.xs, self.ys)
             adj = ctypes.c_void_p()
-            _lib.check(L.pic_adjoint_create(sim.handle, self.checkpoint_every, ctypes.byref(adj)), "pic_adjoint_create")
+            _lib.check(L.pic_adjoint_create_levels(sim.handle, self.checkpoint_every, self.segments_per_outer,
+                                                   ctypes.byref(adj)), "pic_adjoint_create_levels")
             try:
                 _lib.check(L.pic_adjoint_run(adj, _lib.dp(sums), _lib.dp(d_rec), _lib.dp(d_inj)), "pic_adjoint_run")
                 k = ctypes.c_int64()
@@ -96,8 +102,12 @@ class KsModel:
                 bwd = ctypes.c_double()
                 _lib.check(L.pic_adjoint_get_info(adj, ctypes.byref(k), ctypes.byref(nbytes), ctypes.byref(fwd),
                                                   ctypes.byref(bwd)), "pic_adjoint_get_info")
+                m, outer, nseg = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+                _lib.check(L.pic_adjoint_get_levels(adj, ctypes.byref(m), ctypes.byref(outer), ctypes.byref(nseg)),
+                           "pic_adjoint_get_levels")
                 self.last_info = dict(checkpoint_every=k.value, device_bytes=nbytes.value, forward_ms=fwd.value,
-                                      backward_ms=bwd.value)
+                                      backward_ms=bwd.value, segments_per_outer=m.value,
+                                      outer_checkpoints=outer.value, segments=nseg.value)
             finally:
                 L.pic_adjoint_destroy(adj)
         return self._chain(theta, sums, d_rec, d_inj)

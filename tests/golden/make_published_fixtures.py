@@ -6,6 +6,7 @@ Sources, relative to the reference checkout:
   profiling/data/gpu_athena/gpu_5um_full_electrode.json and cpu_5um_full_electrode.json
   profiling/data/gh200/convergence.json               r 0.05 cm at 10/5/2 um: k_s
   profiling/data/gh200/resolution.json                200-step prefix f at 10/5/2/1 um
+  profiling/data/gh200/full_electrode_2.0um.json      500-step prefix f of the 2 um full electrode
   examples/ifj_aic144/README.md (tier table)          dev/archive/standard/wide/full grids and tracks
   examples/ifj_aic144/results/full_electrode/collected_charge.csv  time column (dt)
   web/core/src/theory.rs:76-82                        Jaffe k_s golden for the single-track test scenario
@@ -43,6 +44,11 @@ def main():
         out["prefix_f"].append({"sampled_radius_cm": row["sampled_radius_cm"], "grid_size_um": row["grid_size_um"],
                                 "steps": row["steps"], "f_end": row["f_end"], "no_xy": row["no_xy"],
                                 "no_z_with_buffer": row["no_z_with_buffer"]})
+    for row in rows("profiling/data/gh200/full_electrode_2.0um.json"):
+        out["prefix_f"].append({"sampled_radius_cm": row["sampled_radius_cm"], "grid_size_um": row["grid_size_um"],
+                                "steps": row["steps"], "f_end": row["f_end"], "no_xy": row["no_xy"],
+                                "no_z_with_buffer": row["no_z_with_buffer"],
+                                "total_time_steps": row["total_time_steps"]})
     # examples/ifj_aic144/README.md tier table (name, radius, grid, tracks, rounded k_s)
     out["tiers"] = [
         {"tier": "dev", "sampled_radius_cm": 0.003, "no_xy": 12, "nzb": 210, "tracks": 3157, "ks_4dp": 1.0580},

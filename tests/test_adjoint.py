@@ -3,9 +3,9 @@
 Checked three ways: against the CPU reverse-mode oracle (itself pinned to
 finite differences in test_adjoint_oracle.py), against FP64 central finite
 differences of the GPU forward run (relative step 1e-6, dt and step count
-frozen), and for independence from the checkpoint interval (the recomputed
-segments reproduce the forward states exactly, so every interval must give
-the same numbers bit for bit).
+frozen), and for independence from the checkpoint interval and from the
+number of checkpoint levels (the recomputed segments reproduce the forward
+states exactly, so every choice must give the same numbers bit for bit).
 """
 
 import numpy as np
@@ -77,6 +77,31 @@ def test_checkpoint_interval_does_not_change_the_result(every):
     cfg = SimulationConfig(**{**SMALL, **TWO_SPECIES})
     base = KsModel(cfg, checkpoint_every=16).ks_and_gradient()
     got = KsModel(cfg, checkpoint_every=every).ks_and_gradient()
+    assert got[0] == base[0]
+    np.testing.assert_array_equal(got[1], base[1])
+
+
+@pytest.mark.parametrize("outer", [0, 1, 3, 7, 1000])
+def test_two_level_checkpoints_do_not_change_the_result(outer):
+    """Checkpoints every step give hundreds of segments; keeping only every
+    M-th segment's start and recomputing the spans in the reverse pass (M = 3
+    and 7 leave a short last span; 1000 is one span over the whole run; 0
+    picks M = ceil(sqrt(segments))) must give the one-level numbers bit for
+    bit, and hold fewer states."""
+    cfg = SimulationConfig(**{**SMALL, **TWO_SPECIES})
+    one = KsModel(cfg, checkpoint_every=1, segments_per_outer=1)
+    base = one.ks_and_gradient()
+    assert one.last_info["segments_per_outer"] == 1
+    nseg = one.last_info["segments"]
+    assert nseg == cfg.total_time_steps > 64
+    model = KsModel(cfg, checkpoint_every=1, segments_per_outer=outer)
+    got = model.ks_and_gradient()
+    info = model.last_info
+    m = {0: int(np.ceil(np.sqrt(nseg))), 1000: nseg}.get(outer, outer)
+    assert info["segments_per_outer"] == m
+    assert info["outer_checkpoints"] == -(-nseg // m)
+    if 1 < m < nseg:
+        assert info["device_bytes"] < one.last_info["device_bytes"]
     assert got[0] == base[0]
     np.testing.assert_array_equal(got[1], base[1])
 

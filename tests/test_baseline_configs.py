@@ -19,6 +19,9 @@ Pins, by configuration:
 * The reference's 200-step prefix f at r 0.05 cm for 10, 5, 2 and 1 um
   (profiling/data/gh200/resolution.json); 1 um is 1006^2 x 2010 = 2.03 G
   voxels, 65 GB of FP64 fields.
+* The reference's 500-step prefix f of the 2 um full electrode
+  (profiling/data/gh200/full_electrode_2.0um.json): 2656^2 x 1010 = 7.1 G
+  voxels, 228 GB of FP64 fields.
 
 Bars: FP64 fields 1e-12 relative (k_s, f_t, sums, checksums; fields 1e-12 of
 the peak); FP32 fields |k_s - k_s(FP64)| <= 1e-6, the FP64 value being the
@@ -195,6 +198,26 @@ def test_one_micron_200_step_prefix(precision):
     res = run_simulation_hip(cfg, progress=False, precision=precision, max_steps=row["steps"], return_fields=False)
     f = res.f_t[row["steps"] - 1]
     print(f"1 um 200-step prefix {precision}: f {f!r} published {row['f_end']!r} abs {abs(f - row['f_end']):.2e}")
+    assert res.steps_completed == row["steps"]
+    if precision == "fp64":
+        assert f == pytest.approx(row["f_end"], rel=RTOL)
+    else:
+        assert abs(f - row["f_end"]) <= FP32_DKS
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_two_micron_full_electrode_500_step_prefix(precision):
+    """The 2 um full electrode (2656^2 x 1010 = 7.1 G voxels; 228 GB of FP64
+    fields, held in HBM): the reference published f after its first 500 of
+    13 019 steps, run on its GPU backend over managed memory."""
+    row = next(r for r in PUBLISHED["prefix_f"] if (r["sampled_radius_cm"], r["grid_size_um"]) == (0.265, 2.0))
+    cfg = aic_config(0.265, 2.0, max_voxels=1e10)
+    assert (cfg.no_xy, cfg.no_z_with_buffer, cfg.total_time_steps) == (
+        row["no_xy"], row["no_z_with_buffer"], row["total_time_steps"])
+    res = run_simulation_hip(cfg, progress=False, precision=precision, max_steps=row["steps"], return_fields=False)
+    f = res.f_t[row["steps"] - 1]
+    print(f"2 um full electrode {row['steps']}-step prefix {precision}: f {f!r} published {row['f_end']!r} "
+          f"abs {abs(f - row['f_end']):.2e}")
     assert res.steps_completed == row["steps"]
     if precision == "fp64":
         assert f == pytest.approx(row["f_end"], rel=RTOL)
