@@ -1467,12 +1467,21 @@ int run_steps_on(pic_sim *s, const TrackView &tv, int64_t first, int64_t count, 
     // One segment per gather batch. Launched one by one, a segment's gather
     // runs on the side stream beside the previous segment's sweeps: it starts
     // once that segment's first sweep has folded the step before it, and the
-    // segment's sweeps wait for it. With graph replay a segment's gather and
-    // sweeps are collected, then replayed through graph chains in order.
+    // segment's sweeps wait for it. The first segment of a call has nothing
+    // to run beside, so it is kept to kLeadSteps steps: its short gather is
+    // all the call waits for before its first sweep, and the next segment's
+    // gather runs beside its sweeps. (5 um full electrode, 20 steps from the
+    // start: one 20-step segment waited 0.93 ms for its gather.) With graph
+    // replay a segment's gather and sweeps are collected, then replayed
+    // through graph chains in order.
+    constexpr int64_t kLeadSteps = 2;
     const bool chained = s->graphs && !s->timing && !direct;
-    // the gather of segment [g0, ...) on the side stream: 1 queued, 0 no tracks, else an error
+    auto seg_end = [&](int64_t b0) {
+        return std::min(end, b0 + (b0 == first ? std::min<int64_t>(kLeadSteps, s->batch) : s->batch));
+    };
+    // the gather of segment [g0, seg_end(g0)) on the side stream: 1 queued, 0 no tracks, else an error
     auto side_gather = [&](int64_t g0) -> int {
-        const int64_t g1 = std::min(end, g0 + s->batch);
+        const int64_t g1 = seg_end(g0);
         if (g0 >= end || off[g1] <= off[g0]) return 0;
         int r = enqueue_gather(s, tv.xs, tv.ys, tv.offsets_d, g0, g1 - g0, s->gstream, tv.uploaded ? off : nullptr);
         if (!r && hipEventRecord(s->seg_dens, s->gstream) != hipSuccess) r = fail(PIC_E_HIP, "hipEventRecord failed");
@@ -1484,8 +1493,8 @@ int run_steps_on(pic_sim *s, const TrackView &tv, int64_t first, int64_t count, 
         if (g == 1) gather_queued = true;
         else if (g) rc = g;
     }
-    for (int64_t b0 = first; b0 < end && rc == PIC_OK; b0 += s->batch) {
-        const int64_t b1 = std::min(end, b0 + s->batch);
+    for (int64_t b0 = first, b1 = 0; b0 < end && rc == PIC_OK; b0 = b1) {
+        b1 = seg_end(b0);
         if (chained) {
             s->seg.clear();
             s->sink = &s->seg;
