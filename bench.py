@@ -59,7 +59,7 @@ METRIC = "voxel-steps/s & % HBM BW; batched k_s-curve time vs plain at |dk_s|<=1
 PEAK_GBS = 8000.0
 
 
-def workload(name):
+def workload(name, dose_rate=None):
     from configs import aic_config, readme_config
 
     if name == "c1":
@@ -67,6 +67,9 @@ def workload(name):
     if name == "c0":
         return readme_config(10.0, 0.008), "10um_r0.008cm_readme_example"
     if name == "c4":
+        if dose_rate is not None:
+            return (aic_config(0.265, 5.0, dose_rate_water_Gy_s=dose_rate),
+                    f"5um_full_electrode_r0.265cm_{dose_rate:g}Gy_s")
         return aic_config(0.265, 5.0), "5um_full_electrode_r0.265cm"
     if name == "c5":
         return aic_config(0.265, 2.0, max_voxels=1e10), "2um_full_electrode_r0.265cm"
@@ -370,6 +373,25 @@ def gather_batch_steps(cfg):
     return max(1, min(128, (256 << 20) // (cfg.no_xy * cfg.no_xy * 8)))
 
 
+def deposition_alone(cfg, schedule, xs, ys, precision, steps):
+    """The first ``steps`` steps with graph replay: each segment's bin passes
+    and gather are then launched in the sweeps' own stream, so under
+    rocprofv3 --kernel-trace their durations are the deposition's with the
+    device to itself (in a timed run they share it with the sweeps)."""
+    from pulsed_ion_chamber_hip.solver_hip import HipSimulation
+
+    with HipSimulation(cfg, precision) as sim:
+        sim.set_graphs(True)
+        sim.upload_tracks(schedule, xs, ys)
+        t0 = time.perf_counter()
+        sim.run_steps(0, steps)
+        sim.synchronize()
+        wall = time.perf_counter() - t0
+        per = sim.read_per_step(0, steps)
+    return {"steps": steps, "wall_s": wall, "injected": float(per[:, 0].sum()),
+            "gather_batch_steps": gather_batch_steps(cfg)}
+
+
 def single_run_line(cfg, workload_key, schedule, xs, ys, precision, steps, warmup):
     """Time ``steps`` steps from the start of the run (after ``warmup``
     untimed steps and a reset) with every track resident in HBM; the sweep's
@@ -483,6 +505,11 @@ def main():
     ap.add_argument("--no-mid-size", action="store_true", help="skip the mid-size solve key of the c4 line")
     ap.add_argument("--cpu-budget-s", type=float, default=25.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--dose-rate", type=float, default=None,
+                    help="water dose rate of the c4 workload in Gy/s for --deposition-alone (default: its 10 Gy/s)")
+    ap.add_argument("--deposition-alone", action="store_true",
+                    help="only run --steps steps (default 112) with graph replay, the deposition alone on the device "
+                         "(for rocprofv3 kernel statistics), and print their wall time")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -559,9 +586,17 @@ def main():
         }
         print(json.dumps(out))
         return
-    cfg, wl_name = workload(args.workload)
+    if args.dose_rate is not None and not (args.deposition_alone and args.workload == "c4"):
+        ap.error("--dose-rate applies to the c4 workload with --deposition-alone")
+    cfg, wl_name = workload(args.workload, args.dose_rate)
     total = cfg.total_time_steps
     steps = total if args.steps <= 0 else min(args.steps, total)
+    if args.deposition_alone:
+        n = min(112, total) if args.steps <= 0 else steps
+        schedule, xs, ys = draw_tracks(cfg)
+        out = {"workload": wl_name, "deposition_alone": deposition_alone(cfg, schedule, xs, ys, args.precision, n)}
+        print(json.dumps(out))
+        return
     if args.workload == "c5":
         # the FP64 comparison run would need 228 GB of device memory, the CPU
         # oracle 228 GB of host memory, and the whole FP32 run ~6 minutes:
