@@ -194,7 +194,8 @@ def cpu_baseline(cfg, schedule, xs, ys, budget_s):
 def curve_bench(precision, cpu_budget_s):
     """The 32-point saturation curve: one batch against 32 sequential plain
     runs of the same configurations, tracks already resident in HBM for both
-    (the host draws them once, untimed). Returns the curve's figures."""
+    (the host draws them once, untimed), each side the median of three timed
+    passes. Returns the curve's figures."""
     from configs import saturation_curve_configs
     from pulsed_ion_chamber_hip.batch import HipBatch
     from pulsed_ion_chamber_hip.pulses import draw_tracks
@@ -211,24 +212,30 @@ def curve_bench(precision, cpu_budget_s):
 
     # Both sides are timed in steady state: one whole untimed pass first, so
     # every kernel and tiling either side uses has been launched once (a
-    # kernel's first launch loads its code) and every buffer is allocated.
-    def batched(prec, timed):
+    # kernel's first launch loads its code) and every buffer is allocated;
+    # then PASSES timed passes of each, whose medians are compared (one pass
+    # of the batch varies by about a millisecond from the next).
+    PASSES = 3
+
+    def batched(prec, timed, passes):
         sims = [HipSimulation(c, prec) for c in cfgs]
         for sim, d in zip(sims, draws):
             sim.upload_tracks(*d)
+        loops = []
         with HipBatch(sims) as b:
             b.run()
             b.synchronize()
-            for sim in sims:
-                sim.reset()
-            b.synchronize()
-            t0 = time.perf_counter()
-            b.run()
-            b.synchronize()
-            loop = time.perf_counter() - t0
+            for _ in range(passes):
+                for sim in sims:
+                    sim.reset()
+                b.synchronize()
+                t0 = time.perf_counter()
+                b.run()
+                b.synchronize()
+                loops.append(time.perf_counter() - t0)
             timing = None
             if timed:
-                # kernel times from a second pass with events around every
+                # kernel times from one more pass with events around every
                 # launch (the events cost host time, so not in the wall clock)
                 for sim in sims:
                     sim.reset()
@@ -239,31 +246,37 @@ def curve_bench(precision, cpu_budget_s):
         ks = [ks_of(sim, c) for sim, c in zip(sims, cfgs)]
         for sim in sims:
             sim.close()
-        return loop, ks, timing, tiling
+        return loops, ks, timing, tiling
 
-    def sequential(prec):
-        for c, d in zip(cfgs, draws):  # the untimed pass
-            with HipSimulation(c, prec) as sim:
+    def sequential(prec, passes):
+        sims = [HipSimulation(c, prec) for c in cfgs]
+        try:
+            for sim, c, d in zip(sims, cfgs, draws):  # the untimed pass
                 sim.upload_tracks(*d)
                 sim.run_steps(0, c.total_time_steps)
                 sim.synchronize()
-        total = 0.0
-        ks = []
-        for q, (c, d) in enumerate(zip(cfgs, draws)):
-            with HipSimulation(c, prec) as sim:
-                sim.upload_tracks(*d)
-                sim.synchronize()
-                t0 = time.perf_counter()
-                for first in range(0, c.total_time_steps, 256):  # run_simulation_hip's chunking
-                    sim.run_steps(first, min(256, c.total_time_steps - first))
-                sim.synchronize()
-                total += time.perf_counter() - t0
-                ks.append(ks_of(sim, c))
-        return total, ks
+            totals = []
+            for _ in range(passes):
+                total = 0.0
+                for sim, c in zip(sims, cfgs):
+                    sim.reset()
+                    sim.synchronize()
+                    t0 = time.perf_counter()
+                    for first in range(0, c.total_time_steps, 256):  # run_simulation_hip's chunking
+                        sim.run_steps(first, min(256, c.total_time_steps - first))
+                    sim.synchronize()
+                    total += time.perf_counter() - t0
+                totals.append(total)
+            ks = [ks_of(sim, c) for sim, c in zip(sims, cfgs)]
+        finally:
+            for sim in sims:
+                sim.close()
+        return totals, ks
 
-    loop_b, ks_b, timing, tiling = batched(precision, True)
-    loop_s, ks_s = sequential(precision)
-    _, ks64, _, _ = batched("fp64", False)
+    loops_b, ks_b, timing, tiling = batched(precision, True, PASSES)
+    loops_s, ks_s = sequential(precision, PASSES)
+    loop_b, loop_s = float(np.median(loops_b)), float(np.median(loops_s))
+    _, ks64, _, _ = batched("fp64", False, 0)
     bytes_per_voxel = 16 if precision == "fp32" else 32
     sweep_bytes = sum(vsteps) * bytes_per_voxel
     out = {
@@ -276,6 +289,9 @@ def curve_bench(precision, cpu_budget_s):
         "batched_s": loop_b,
         "sequential_plain_s": loop_s,
         "speedup": loop_s / loop_b,
+        "timed_passes": PASSES,
+        "batched_s_passes": loops_b,
+        "sequential_plain_s_passes": loops_s,
         "voxel_steps_per_s": sum(vsteps) / loop_b,
         "batched_sweep_ms": timing["sweep_ms"],
         "batched_sweep_launches": timing["sweep_launches"],
