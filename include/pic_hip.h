@@ -246,7 +246,9 @@ int pic_batch_set_graphs(pic_batch *batch, int enable);
  *           n_cen, alpha_dt, gaussian_factor, h2_over_b2) -- the pic_params
  *           fields, the deposition box held fixed,
  *   d_inj = d(sum injected) / d(gaussian_factor, h2_over_b2).
- * k_s = sum_inj / (sum_inj - sum_rec) and its derivatives follow on the host. */
+ * k_s = sum_inj / (sum_inj - sum_rec) and its derivatives follow on the host.
+ * Afterwards the handle's per-step record is the run's; its fields are not
+ * the final state (the reverse pass reruns parts of the run in them). */
 int pic_adjoint_create(pic_sim *sim, int64_t checkpoint_every, pic_adjoint **out);
 /* As pic_adjoint_create, with the checkpoints in two levels: the forward pass
  * keeps the state at the start of every segments_per_outer-th segment only,
