@@ -24,7 +24,9 @@ over r 0.05 cm, 208 x 208 x 406, 3388 steps, seed 1; also reported as the
 grids 16^2..47^2 x 106..206, 243..2505 steps) timed as one batch against 32
 sequential plain runs of the same configurations (the ``curve`` key of the
 default line); ``c3`` the adjoint (dk_s/dtheta over 3 pulses of a 48^2 x 206
-grid, FP64) with its Richardson-extrapolated finite-difference check.
+grid, FP64) with its Richardson-extrapolated finite-difference check;
+``published`` every run the reference publishes a wall time for, rerun whole
+in FP64 and FP32 beside that time and its published k_s or f.
 
 Prints one JSON line. ``value`` is interior voxel-steps per second over the
 timed region; ``roofline`` divides the sweep kernel's compulsory HBM bytes
@@ -82,6 +84,11 @@ TIMING_GROUP = 16
 
 def interior_voxels(cfg):
     return (cfg.no_xy - 2) ** 2 * (cfg.no_z_with_buffer - 2)
+
+
+def interior_voxels_of(run):
+    nx, _, nz = run["grid"]
+    return (nx - 2) ** 2 * (nz - 2)
 
 
 def gpu_run(cfg, schedule, xs, ys, precision, steps, warmup):
@@ -329,6 +336,56 @@ def curve_bench(precision, cpu_budget_s):
     return out
 
 
+def published_runs_bench(precisions=("fp64", "fp32")):
+    """Every run the reference publishes a wall time for (the 10 um ladder
+    and the 5 um full electrode on an A100, r 0.05 cm at 10, 5 and 2 um on a
+    GH200, and the GH200's 200-step resolution ladder and 500 steps of the
+    2 um full electrode), rerun here with run_simulation_hip. Like the
+    reference's wall_s, each time is the whole call -- host track draw,
+    allocation, upload, the steps and the read-back -- after one warm-up
+    call that loads the kernels."""
+    from configs import aic_config
+    from pulsed_ion_chamber_hip import run_simulation_hip
+
+    table = json.loads((ROOT / "tests" / "golden" / "published.json").read_text())
+    cases, seen = [], set()
+    for row in table["aic_runs"]:
+        key = (row["sampled_radius_cm"], row["grid_size_um"])
+        if "gpu_wall_s" in row and key not in seen:
+            seen.add(key)
+            cases.append((row, None))
+    for row in table["prefix_f"]:
+        if "gpu_wall_s" in row:
+            cases.append((row, row["steps"]))
+    run_simulation_hip(aic_config(0.014, 10.0), progress=False, return_fields=False)  # warm-up
+    out = []
+    for row, max_steps in cases:
+        cfg = aic_config(row["sampled_radius_cm"], row["grid_size_um"], max_voxels=1e10)
+        steps = max_steps or cfg.total_time_steps
+        rec = {"sampled_radius_cm": row["sampled_radius_cm"], "grid_size_um": row["grid_size_um"],
+               "grid": [cfg.no_xy, cfg.no_xy, cfg.no_z_with_buffer], "steps": steps,
+               "reference_wall_s": row["gpu_wall_s"], "reference_gpu": row["gpu"]}
+        if "cpu_wall_s" in row:
+            rec["reference_cpu_wall_s"] = row["cpu_wall_s"]
+        for prec in precisions:
+            print(f"bench: published run r {row['sampled_radius_cm']} cm @ {row['grid_size_um']} um, {steps} steps, "
+                  f"{prec}", file=sys.stderr, flush=True)
+            t0 = time.perf_counter()
+            res = run_simulation_hip(cfg, progress=False, precision=prec, return_fields=False, max_steps=max_steps)
+            wall = time.perf_counter() - t0
+            rec[f"{prec}_wall_s"] = wall
+            rec[f"{prec}_speedup_vs_reference"] = row["gpu_wall_s"] / wall
+            if max_steps:
+                f = float(res.f_t[steps - 1])
+                rec[f"{prec}_f_end"] = f
+                rec[f"{prec}_abs_err_f_end"] = abs(f - row["f_end"])
+            else:
+                rec[f"{prec}_ks"] = float(res.ks)
+                rec[f"{prec}_abs_err_ks"] = abs(float(res.ks) - row["gpu_ks"])
+        out.append(rec)
+    return out
+
+
 def adjoint_bench(fd: bool = True, pulses: int = 3):
     """dk_s/dtheta on the 10 um, r 0.02 cm grid over 3 pulses (FP64 fields,
     segments of 64 steps, their starts kept in two levels): forward and
@@ -512,7 +569,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=0, help="timed steps (default: the whole run)")
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--workload", default="c4", choices=["c0", "c1", "c2", "c3", "c4", "c5"])
+    ap.add_argument("--workload", default="c4", choices=["c0", "c1", "c2", "c3", "c4", "c5", "published"])
     ap.add_argument("--no-curve", action="store_true", help="skip the saturation-curve comparison")
     ap.add_argument("--adjoint-pulses", type=int, default=3, help="pulses of the adjoint workload (c3)")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
@@ -574,6 +631,30 @@ def main():
                 "sample": f"first {co['configs']} of 32 curve configurations, whole runs, FP64 fields, "
                           f"{co['threads']} OpenMP threads, {co['seconds']:.1f} s",
             }
+        print(json.dumps(out))
+        return
+    if args.workload == "published":
+        print(f"bench: the reference's published runs, device {device}", file=sys.stderr, flush=True)
+        runs = published_runs_bench()
+        vs = sum(interior_voxels_of(r) * r["steps"] for r in runs)
+        wall64 = sum(r["fp64_wall_s"] for r in runs)
+        out = {
+            "metric": METRIC,
+            "value": vs / wall64,
+            "unit": "voxel-steps/s",
+            "n_gpus": 1,
+            "steps": sum(r["steps"] for r in runs),
+            "warmup": 1,
+            "ms_per_step": wall64 / sum(r["steps"] for r in runs) * 1e3,
+            "higher_is_better": True,
+            "scaling": "weak",
+            "vs_baseline": None,
+            "dtype": "f64",
+            "data": "synthetic",
+            "config": {"workload": "published_runs", "runs": len(runs),
+                       "timing": "whole run_simulation_hip call, as the reference's wall_s"},
+            "runs": runs,
+        }
         print(json.dumps(out))
         return
     if args.workload == "c3":

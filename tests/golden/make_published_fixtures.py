@@ -28,27 +28,38 @@ def rows(path):
 
 
 def main():
+    # wall times: the whole run_simulation_* call, FP64 fields (the GPU rows
+    # on one A100-SXM4-40GB or one GH200, the CPU rows on 32 cores of an
+    # Athena node, docs/GPU.md:22-30 and docs/BENCHMARKS-HELIOS-GH200.md)
+    a100, gh200, cpu32 = "A100-SXM4-40GB (Numba-CUDA)", "GH200 (Numba-CUDA, device memory)", "32 CPU cores (batched)"
     out = {"aic_runs": [], "prefix_f": [], "tiers": [], "sources": __doc__}
     for row in rows("profiling/data/gpu_athena/ladder_10um.json"):
         out["aic_runs"].append({k: row[k] for k in ("sampled_radius_cm", "grid_size_um", "no_xy", "no_z_with_buffer",
-                                                    "tracks_per_pulse", "total_time_steps", "cpu_ks", "gpu_ks")})
+                                                    "tracks_per_pulse", "total_time_steps", "cpu_ks", "gpu_ks")}
+                               | {"gpu_wall_s": row["gpu_wall_s"], "gpu": a100, "cpu_wall_s": row["cpu_wall_s"],
+                                  "cpu": cpu32})
     gpu = rows("profiling/data/gpu_athena/gpu_5um_full_electrode.json")[0]
     cpu = rows("profiling/data/gpu_athena/cpu_5um_full_electrode.json")[0]
     out["aic_runs"].append({k: gpu[k] for k in ("sampled_radius_cm", "grid_size_um", "no_xy", "no_z_with_buffer",
-                                                "tracks_per_pulse", "total_time_steps", "gpu_ks")} | {"cpu_ks": cpu["cpu_ks"]})
+                                                "tracks_per_pulse", "total_time_steps", "gpu_ks")}
+                           | {"cpu_ks": cpu["cpu_ks"], "gpu_wall_s": gpu["gpu_wall_s"], "gpu": a100,
+                              "cpu_wall_s": cpu["cpu_wall_s"], "cpu": cpu32})
     for row in rows("profiling/data/gh200/convergence.json"):
         out["aic_runs"].append({"sampled_radius_cm": row["sampled_radius_cm"], "grid_size_um": row["grid_size_um"],
                                 "no_xy": row["no_xy"], "no_z_with_buffer": row["no_z_with_buffer"],
-                                "total_time_steps": row["total_time_steps"], "gpu_ks": row["ks"]})
+                                "total_time_steps": row["total_time_steps"], "gpu_ks": row["ks"],
+                                "gpu_wall_s": row["wall_s"], "gpu": gh200})
     for row in rows("profiling/data/gh200/resolution.json"):
         out["prefix_f"].append({"sampled_radius_cm": row["sampled_radius_cm"], "grid_size_um": row["grid_size_um"],
                                 "steps": row["steps"], "f_end": row["f_end"], "no_xy": row["no_xy"],
-                                "no_z_with_buffer": row["no_z_with_buffer"]})
+                                "no_z_with_buffer": row["no_z_with_buffer"], "gpu_wall_s": row["wall_s"],
+                                "gpu": gh200})
     for row in rows("profiling/data/gh200/full_electrode_2.0um.json"):
         out["prefix_f"].append({"sampled_radius_cm": row["sampled_radius_cm"], "grid_size_um": row["grid_size_um"],
                                 "steps": row["steps"], "f_end": row["f_end"], "no_xy": row["no_xy"],
                                 "no_z_with_buffer": row["no_z_with_buffer"],
-                                "total_time_steps": row["total_time_steps"]})
+                                "total_time_steps": row["total_time_steps"], "gpu_wall_s": row["wall_s"],
+                                "gpu": "GH200 (Numba-CUDA, managed memory: the fields exceed its HBM)"})
     # examples/ifj_aic144/README.md tier table (name, radius, grid, tracks, rounded k_s)
     out["tiers"] = [
         {"tier": "dev", "sampled_radius_cm": 0.003, "no_xy": 12, "nzb": 210, "tracks": 3157, "ks_4dp": 1.0580},
