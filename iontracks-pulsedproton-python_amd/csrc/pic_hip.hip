@@ -234,8 +234,7 @@ int sm_count(int device) {
 // k values per lane: FP64 fields one (8-byte loads); FP32 fields one, two or
 // four (4-, 8- or 16-byte loads). FP32 grids take four when four-value windows
 // take no more lanes than two-value ones and 2-row tiles of 16 planes give ~16
-// waves per CU of MI355X (256 CUs; fixed, since the row pitch follows from
-// this choice before any device is opened), and two otherwise. Measured on
+// waves per CU of MI355X (256 CUs), and two otherwise. Measured on
 // MI355X: 1066^2 x 410, four values 1.68 ms against 1.74 (each at its best
 // pitch); 208^2 x 406 (a third of the waves), 78.5 against 78.7 us at the
 // same pitch; the 16^2..47^2 x 106..206 grids of a saturation curve run one
@@ -254,7 +253,7 @@ int values_per_lane(const pic_sim *s, int force_v) {
     if (s->prec == PIC_FP64) return 1;
     if (force_v > 0) return force_v;
     if (s->force_tj > 2) return 2;  // tiles of four rows exist for one and two values
-    return four_values_suit(&s->p) && s->pitch % 4 == 0 ? 4 : 2;
+    return four_values_suit(&s->p) ? 4 : 2;
 }
 
 // Fixed tiling of one configuration: v values per lane, tj rows per tile,
@@ -894,19 +893,17 @@ void swap_buffers(pic_sim *s) {
     std::swap(s->cur_n, s->nxt_n);
 }
 
-// elements from one (i, j) row to the next: FP32 rows are padded to an even
-// length so that two-value lanes load 8-byte aligned pairs
-// FP32 rows are padded so every row is aligned for the layout's loads: to a
-// multiple of four values (16 bytes) where the grid takes four values per
-// lane, else to an even length (8 bytes). The pitch also sets how rows and
-// planes fall on HBM channels, and rows padded further than needed measured
-// slower: 1066^2 x 410 with two values per lane 1.74 ms at pitch 410, 1.83 at
-// 412 and 416, 1.98 at 448; 2656^2 x 1010, 26.9 ms at 1010 against 40.8 at
-// 1012.
+// Elements from one (i, j) row to the next: FP32 rows are padded to an even
+// length, so every lane's first value is 8-byte aligned (four-value lanes
+// then load 16 bytes at 8-byte alignment, which buffer loads allow). The
+// pitch also sets how rows and planes fall on HBM channels, and every padding
+// beyond the least measured slower: 1066^2 x 410 with four values per lane
+// 1.43-1.48 ms at pitch 410 against 1.46-1.52 at 412 (same box), 1.72 at
+// 420, 1.80 at 428, 1.92-1.96 at 436..456; with two values 1.74 ms at 410,
+// 1.83 at 412 and 416; 2656^2 x 1010, 26.9 ms at 1010 against 40.8 at 1012.
 int64_t row_pitch(const pic_params *p, int precision) {
     if (precision != PIC_FP32) return p->nzb;
-    const int64_t m = four_values_suit(p) ? 4 : 2;
-    return (p->nzb + m - 1) / m * m;
+    return (p->nzb + 1) / 2 * 2;
 }
 
 int check_params(const pic_params *p, int precision) {
@@ -920,7 +917,7 @@ int check_params(const pic_params *p, int precision) {
     // the sweep addresses one i plane of an array through a buffer resource
     // with 32-bit offsets
     const int64_t plane_bytes = p->no_xy * row_pitch(p, precision) * (precision == PIC_FP64 ? 8 : 4);
-    if (plane_bytes >= INT32_MAX)
+    if (plane_bytes + pic::kFieldSlack >= INT32_MAX)
         return fail(PIC_E_INVALID, "one i plane (%lld x %lld) exceeds 2 GB", (long long)p->no_xy, (long long)p->nzb);
     return PIC_OK;
 }
@@ -1277,7 +1274,7 @@ int pic_mem_required(const pic_params *p, int precision, size_t *bytes) {
     const size_t xy = (size_t)p->no_xy * p->no_xy;
     const size_t gt = (size_t)ceil_div(p->no_xy, pic::kGT);
     const size_t b = (size_t)dens_slots(batch_steps(p));
-    *bytes = 4 * vox * elem + b * (xy * 8 + gt * gt * 8) + xy * 8 + 2 * partial_cap_bytes(p) + 256;
+    *bytes = 4 * (vox * elem + pic::kFieldSlack) + b * (xy * 8 + gt * gt * 8) + xy * 8 + 2 * partial_cap_bytes(p) + 256;
     return PIC_OK;
 }
 
@@ -1316,8 +1313,11 @@ int pic_create(const pic_params *p, int precision, pic_sim **out) {
         return cleanup(fail(PIC_E_HIP, "hipEventCreate failed"));
     void **bufs[4] = {&s->cur_p, &s->cur_n, &s->nxt_p, &s->nxt_n};
     for (void **b : bufs) {
-        if (hipMalloc(b, s->n_vox * s->elem) != hipSuccess) return cleanup(fail(PIC_E_NOMEM, "hipMalloc of a carrier array failed"));
-        if (hipMemsetAsync(*b, 0, s->n_vox * s->elem, s->stream) != hipSuccess) return cleanup(fail(PIC_E_HIP, "hipMemsetAsync failed"));
+        // kFieldSlack zero bytes past the last row: a four-value lane at the
+        // end of a row whose length is 2 mod 4 loads 8 bytes past it
+        const size_t bytes = s->n_vox * s->elem + pic::kFieldSlack;
+        if (hipMalloc(b, bytes) != hipSuccess) return cleanup(fail(PIC_E_NOMEM, "hipMalloc of a carrier array failed"));
+        if (hipMemsetAsync(*b, 0, bytes, s->stream) != hipSuccess) return cleanup(fail(PIC_E_HIP, "hipMemsetAsync failed"));
     }
     const size_t xy = (size_t)p->no_xy * p->no_xy;
     s->sweep_part_cap = (int64_t)(partial_cap_bytes(p) / (3 * sizeof(double)));
@@ -1756,9 +1756,6 @@ int pic_set_tiling(pic_sim *s, int rows_per_tile, int planes_per_chunk, int valu
         return fail(PIC_E_INVALID, "%d values per lane needs FP32 fields", values_per_lane);
     if (values_per_lane == 4 && rows_per_tile > 2)
         return fail(PIC_E_INVALID, "four values per lane takes rows_per_tile 0, 1 or 2");
-    if (values_per_lane == 4 && s->pitch % 4 != 0)
-        return fail(PIC_E_INVALID, "four values per lane needs rows padded to a multiple of four values; this grid's rows "
-                    "hold %lld (four values are used by default where they pay)", (long long)s->pitch);
     if (rows_per_tile && rows_per_tile != 1 && rows_per_tile != 2 && rows_per_tile != 4 && rows_per_tile != 8)
         return fail(PIC_E_INVALID, "rows_per_tile must be 0, 1, 2, 4 or 8");
     if (rows_per_tile == 8 && (s->prec != PIC_FP32 || values_per_lane != 1))

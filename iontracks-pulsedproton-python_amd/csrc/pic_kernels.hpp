@@ -2,9 +2,9 @@
 //
 // Grid layout: each species is (no_xy, no_xy, nzb), k fastest, stored as S
 // (double or float) with a row pitch of `pitch` >= nzb elements (FP32 rows are
-// padded to an even length, or to a multiple of four where the sweep takes
-// four values per lane, so every row is aligned for the lane's loads; the
-// padding is zero and only ever written with zeros). Interior voxels are
+// padded to an even length, so every lane's first value is 8-byte aligned;
+// the padding is zero and only ever written with zeros), followed by
+// kFieldSlack zero bytes. Interior voxels are
 // 1 <= i,j <= no_xy-2,
 // 1 <= k <= nzb-2.
 #pragma once
@@ -20,6 +20,14 @@ namespace pic {
 
 constexpr int kWave = 64;
 constexpr int kOut = 62;  // output lanes per wave window (lanes 1..62)
+
+// Zero bytes after each carrier array, and past each i plane in the sweep's
+// buffer resources: with four values per lane on rows whose length is 2 mod
+// 4, the lane holding a row's last two values loads 16 bytes, 8 of them past
+// the row (in the next row, or past the plane and the array on their last
+// row). Those two values are never used; the slack keeps the load inside the
+// allocation and the resource's range.
+constexpr int kFieldSlack = 16;
 
 // k values a wave window of v values per lane writes: those of lanes 1..62,
 // so windows step by 62 v and overlap by 2 v, and every storing lane writes
@@ -312,7 +320,10 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &a, const int logical
         const int no_xy = a.no_xy, nzb = a.nzb, pitch = a.pitch;
         const int step = V == 2 ? a.win_step : WIN;
         const int k0 = win * step + V * lane;
-        const bool k_in = k0 < nzb;  // k0 is a multiple of V, and so is the pitch
+        // k0 is a multiple of V; the pitch is a multiple of two (FP32), so a
+        // lane with k0 < nzb reads at most the row's padding or, with four
+        // values on a row of 2 mod 4 values, two values past the row
+        const bool k_in = k0 < nzb;
         bool k_out[V], k_gap[V], k_score[V];
 #pragma unroll
         for (int v = 0; v < V; ++v) {
@@ -374,7 +385,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &a, const int logical
         auto plane_rsrc = [&](const void *arr, int i) {
             return __builtin_amdgcn_make_buffer_rsrc(
                 const_cast<void *>(static_cast<const void *>(static_cast<const S *>(arr) + (int64_t)i * plane)), 0,
-                plane_bytes, kBufferWord3);
+                plane_bytes + kFieldSlack, kBufferWord3);
         };
         auto row_off = [&](int j) -> int { return (int)((int64_t)j * pitch * (int64_t)sizeof(S)); };
         auto load_raw = [&](int i, int j, S (&vp)[V], S (&vn)[V]) {
@@ -457,7 +468,9 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &a, const int logical
             const int so = row_off(j);
             if constexpr (V == 4) {
                 // all four values: one the lane does not own is k = 0, a k end
-                // plane or the row padding, all of which stay zero
+                // plane or the row padding, all of which stay zero; the lane
+                // holding a row's last two values (rows of 2 mod 4 values)
+                // writes those two only
                 if (k_out[0] || k_out[1] || k_out[2] || k_out[3]) {
                     f32x4 wp, wn;
 #pragma unroll
@@ -465,8 +478,14 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &a, const int logical
                         wp[v] = k_out[v] ? vp[v] : 0.0f;
                         wn[v] = k_out[v] ? vn[v] : 0.0f;
                     }
-                    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, wp), rp, vo_store, so, 0);
-                    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, wn), rn, vo_store, so, 0);
+                    if (k0 + 4 <= pitch) {
+                        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, wp), rp, vo_store, so, 0);
+                        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, wn), rn, vo_store, so, 0);
+                    } else {
+                        const f32x2 hp = {wp[0], wp[1]}, hn = {wn[0], wn[1]};
+                        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, hp), rp, vo_store, so, 0);
+                        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, hn), rn, vo_store, so, 0);
+                    }
                 }
                 return;
             }
