@@ -14,7 +14,6 @@ Any object exposing the same attributes (the reference's own
 this class exists so the backend does not need the reference installed.
 """
 
-import os
 import warnings
 from dataclasses import dataclass
 from math import exp, pi, sqrt
@@ -42,8 +41,8 @@ def von_neumann_dt(ion_diff, grid_spacing_cm, ion_mobility, Efield_V_cm):
     mobility = np.atleast_1d(ion_mobility)
     if len(diffusion) != len(mobility):
         raise ValueError(
-            f"ion_diff and ion_mobility must describe the same species: got "
-            f"{len(diffusion)} diffusion coefficient(s) and {len(mobility)} mobility/ies."
+            f"ion_diff has {len(diffusion)} entries and ion_mobility {len(mobility)}: "
+            "give one diffusion coefficient per mobility, species by species."
         )
     dt = 1.0
     while True:
@@ -153,7 +152,7 @@ class SimulationConfig:
             ("D_negative_cm2_s", self.D_negative),
         ):
             if value <= 0:
-                raise ValueError(f"{name} must be a positive magnitude, got {value!r}.")
+                raise ValueError(f"{name} is {value!r}: transport coefficients are magnitudes and must be > 0.")
         self.two_carrier_species = not (
             self.mu_positive == self.mu_negative and self.D_positive == self.D_negative
         )
@@ -161,13 +160,13 @@ class SimulationConfig:
     def _validate_options(self):
         if self.lateral_boundary not in LATERAL_BOUNDARIES:
             raise ValueError(
-                f"lateral_boundary must be one of {LATERAL_BOUNDARIES}, got {self.lateral_boundary!r}."
+                f"unknown lateral_boundary {self.lateral_boundary!r} (choose from {LATERAL_BOUNDARIES})."
             )
         if self.scoring_region not in SCORING_REGIONS:
-            raise ValueError(f"scoring_region must be one of {SCORING_REGIONS}, got {self.scoring_region!r}.")
+            raise ValueError(f"unknown scoring_region {self.scoring_region!r} (choose from {SCORING_REGIONS}).")
         fill = self.chamber_fill_fraction
         if isinstance(fill, bool) or not 0 < fill <= 1:
-            raise ValueError(f"chamber_fill_fraction must be a number in (0, 1], got {fill!r}.")
+            raise ValueError(f"chamber_fill_fraction {fill!r} is outside (0, 1]: it is the filled share of the chamber.")
 
     # --- grid (config.py:200-284) ------------------------------------------
     def _derive_grid(self):
@@ -180,16 +179,16 @@ class SimulationConfig:
         self._gap_mismatch_relative = mismatch / self.electrode_gap_cm if self.electrode_gap_cm > 0 else 0.0
         if self._gap_mismatch_relative > 1e-3:
             warnings.warn(
-                f"electrode_gap_cm={self.electrode_gap_cm:.6g} cm is not an integer multiple of "
-                f"grid_size_um={self.grid_size_um:.4g} um: the simulated gap is no_z={self.no_z} voxels = "
-                f"{self.effective_gap_cm:.6g} cm, while Efield_V_cm keeps the requested gap.",
+                f"electrode_gap_cm={self.electrode_gap_cm:.6g} cm rounds to no_z={self.no_z} voxels of "
+                f"{self.grid_size_um:.4g} um = {self.effective_gap_cm:.6g} cm; the field Efield_V_cm still "
+                "uses the requested gap.",
                 stacklevel=3,
             )
         if self.no_xy * self.no_xy * self.no_z > self.max_voxels:
             raise ValueError(
-                f"Grid too large: {self.no_xy}x{self.no_xy}x{self.no_z} "
-                f"({self.no_xy * self.no_xy * self.no_z:.3g} voxels). "
-                "Increase grid_size_um or reduce sampled_radius_cm."
+                f"{self.no_xy}x{self.no_xy}x{self.no_z} = {self.no_xy * self.no_xy * self.no_z:.3g} voxels "
+                f"exceeds max_voxels={self.max_voxels:.3g}; a coarser grid_size_um or a smaller "
+                "sampled_radius_cm brings it down."
             )
         # The centre is the continuous midpoint no_xy/2.0, kept as a float.
         self.outer_radius = self.no_xy / 2.0
@@ -197,9 +196,9 @@ class SimulationConfig:
         self.inner_radius = self.outer_radius - self.buffer_radius
         if self.inner_radius <= 0:
             raise ValueError(
-                f"inner_radius = {self.inner_radius:.3g} <= 0 (no_xy={self.no_xy}, "
-                f"buffer_radius={self.buffer_radius}): sampled_radius_cm is too small "
-                "relative to buffer_radius/grid_size_um."
+                f"no sampled disc is left: inner_radius {self.inner_radius:.3g} voxels once the "
+                f"{self.buffer_radius}-voxel buffer is taken from no_xy={self.no_xy}; raise sampled_radius_cm "
+                "or lower grid_size_um or buffer_radius."
             )
         self.inner_radius_sq = self.inner_radius**2
         self.sampling_radius = self.inner_radius * self.chamber_fill_fraction
@@ -220,8 +219,8 @@ class SimulationConfig:
             self.dt = self.von_neumann_dt
         elif self.dt_seconds > self.von_neumann_dt:
             raise ValueError(
-                f"dt_seconds={self.dt_seconds:.4g} s exceeds the von Neumann stability "
-                f"limit of {self.von_neumann_dt:.4g} s for this grid and field."
+                f"dt_seconds={self.dt_seconds:.4g} s is above {self.von_neumann_dt:.4g} s, the largest step "
+                "the explicit scheme stays stable at on this grid and field."
             )
         else:
             self.dt = float(self.dt_seconds)
@@ -231,8 +230,8 @@ class SimulationConfig:
         self.rf_cycles_per_time_step = self.dt * self.rf_frequency_hz if self.rf_frequency_hz else None
         if self.rf_cycles_per_time_step is not None and self.rf_cycles_per_time_step < 1.0:
             warnings.warn(
-                f"dt ({self.dt:.3g} s) is shorter than one RF period "
-                f"(1/{self.rf_frequency_hz:.4g} Hz): RF bucket structure may not be safely averaged over.",
+                f"dt = {self.dt:.3g} s resolves the RF period (1/{self.rf_frequency_hz:.4g} Hz), so the "
+                "time-averaged beam may misrepresent the bucket structure.",
                 stacklevel=3,
             )
         self.slowest_mobility_cm2_Vs = min(self.mu_positive, self.mu_negative)
@@ -248,9 +247,8 @@ class SimulationConfig:
             self.pulse_period_steps = self.pulse_time_steps
         if self.n_pulses > 1 and self.pulse_period_steps < self.pulse_time_steps:
             raise ValueError(
-                "Pulse period is shorter than the pulse itself at this dt "
-                f"({self.pulse_period_steps} < {self.pulse_time_steps} steps); "
-                "reduce pulse_duration_s or repetition_rate_hz."
+                f"one pulse takes {self.pulse_time_steps} steps but pulses repeat every "
+                f"{self.pulse_period_steps} at this dt: shorten pulse_duration_s or lower repetition_rate_hz."
             )
         self.total_time_steps = (
             (self.n_pulses - 1) * self.pulse_period_steps + self.pulse_time_steps + self.clearance_time_steps
@@ -268,8 +266,8 @@ class SimulationConfig:
         else:
             if self.track_cutoff_sigmas <= 0:
                 raise ValueError(
-                    f"track_cutoff_sigmas must be positive (or None for no truncation), "
-                    f"got {self.track_cutoff_sigmas!r}."
+                    f"track_cutoff_sigmas {self.track_cutoff_sigmas!r} is not a positive width "
+                    "(None keeps every Gaussian tail)."
                 )
             self.track_cutoff_cm = self.track_cutoff_sigmas * self.track_sigma_cm
             self.track_cutoff_voxels = self.track_cutoff_cm / self.unit_length_cm
@@ -293,9 +291,7 @@ class SimulationConfig:
         if fraction is None:
             return
         if not 0 < fraction <= 1:
-            raise ValueError(f"budget_fraction must be in (0, 1], got {fraction!r}.")
-        if os.environ.get("PIC_HIP_SKIP_HOST_MEMORY_CHECK") == "1":
-            return
+            raise ValueError(f"budget_fraction {fraction!r} is outside (0, 1].")
         available = _available_host_bytes()
         if available is None or self.estimated_memory_bytes <= available * fraction:
             return
