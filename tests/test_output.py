@@ -27,6 +27,10 @@ def test_record_round_trip_in_ion_pairs(tmp_path):
     assert header == "time,n_positive,n_negative,injected_positive,injected_negative,recombination"
     first = paths["csv"].read_text().splitlines()[1].split(",")
     assert all("e" in v and len(v.split("e")[0].split(".")[1]) == 12 for v in first)
+    # the reference writes with Python's csv module: every line ends in CR LF
+    raw = paths["csv"].read_bytes().split(b"\n")
+    assert raw[-1] == b"" and all(line.endswith(b"\r") for line in raw[:-1])
+    assert len(raw) - 1 == 1 + len(result.time_s)
 
 
 def test_missing_file_is_named(tmp_path):
