@@ -163,9 +163,10 @@ int pic_get_gather_mode(pic_sim *sim, int *binned);
 /* Block until all work queued on the handle's stream is done. */
 int pic_synchronize(pic_sim *sim);
 
-/* Sweep launch shape, for tuning: rows per tile (1, 2, 4, 8), planes per
- * block chunk, and k values per lane (1; 2 or 4 for FP32 fields, four with
- * rows per tile 1 or 2 only); 0 = automatic. Takes effect at the next launch. */
+/* Sweep launch shape, for tuning: rows per tile (1, 2, 3, 4, 8), planes per
+ * block chunk, and k values per lane (1; 2 or 4 for FP32 fields: four with
+ * rows per tile 1, 2 or 3 only, three rows with four values only, eight rows
+ * with one value only); 0 = automatic. Takes effect at the next launch. */
 int pic_set_tiling(pic_sim *sim, int rows_per_tile, int planes_per_chunk, int values_per_lane);
 
 /* Consecutive j tiles per sweep block (0 = automatic, at most 8 waves per

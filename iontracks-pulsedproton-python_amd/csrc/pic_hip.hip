@@ -231,29 +231,61 @@ int sm_count(int device) {
     return v;
 }
 
+// Waves of a sweep tiling: blocks of `tpb` tiles of `tj` rows times `wpb`
+// windows, over every (window group, tile group, chunk of `chunk` planes).
+int64_t sweep_waves(int64_t lateral, int64_t n_windows, int tj, int tpb, int chunk) {
+    const int64_t wpb = std::min<int64_t>(n_windows, 8);
+    return ceil_div(n_windows, wpb) * ceil_div(ceil_div(lateral, tj), tpb) * ceil_div(lateral, chunk) * wpb * tpb;
+}
+
 // k values per lane: FP64 fields one (8-byte loads); FP32 fields one, two or
-// four (4-, 8- or 16-byte loads). FP32 grids take four when four-value windows
-// take no more lanes than two-value ones and 2-row tiles of 16 planes give ~16
-// waves per CU of MI355X (256 CUs), and two otherwise. Measured on
-// MI355X: 1066^2 x 410, four values 1.68 ms against 1.74 (each at its best
-// pitch); 208^2 x 406 (a third of the waves), 78.5 against 78.7 us at the
-// same pitch; the 16^2..47^2 x 106..206 grids of a saturation curve run one
-// after another, 108 against 84 ms; 2656^2 x 1010 (nine two-value windows,
-// five four-value ones), 30.2 against 26.9 ms.
-bool four_values_suit(const pic_params *p) {
-    constexpr int64_t kCus = 256;
+// four (4-, 8- or 16-byte loads). FP32 grids take four when four-value
+// windows take no more lanes than two-value ones and the four-value tiling
+// (3-row tiles, two per block) gives at least 6 waves per CU at 16-plane
+// chunks; two otherwise. Measured on MI355X, each pair on one device: 206^2
+// x 410 and 208^2 x 406, four values (3-row pairs) 67.8 and 69.7 us against
+// 72.9 and 74.3 for the two-value tiling (4-row tiles of 8 planes); 166^2 and
+// 246^2 x 210 (one window), two values 27.1 and 50.8 us against 28.6 and
+// 52.0; 2656^2 x 1010 (nine two-value windows, five four-value ones), 26.9
+// against 30.2 ms; the 16^2..47^2 x 106..206 grids of a saturation curve run
+// one after another, 84 against 108 ms. Grids that took four values with
+// 2-row tiles are 1-9 % faster with 3-row ones (compute_sweep_shape).
+bool four_values_suit(const pic_params *p, int cus) {
     const int64_t n = p->nzb - 2, w4 = pic::window_outputs(4), w2 = pic::window_step(2, n);
     const int64_t lateral = p->no_xy - 2;
     const bool lanes_ok = ceil_div(n, w4) * w4 <= ceil_div(n, w2) * w2;
-    const bool waves_ok = ceil_div(lateral, 2) * ceil_div(n, w4) * ceil_div(lateral, 16) >= kCus * 16;
+    const bool waves_ok = sweep_waves(lateral, ceil_div(n, w4), 3, 2, 16) >= (int64_t)cus * 6;
     return lanes_ok && waves_ok;
 }
 
 int values_per_lane(const pic_sim *s, int force_v) {
     if (s->prec == PIC_FP64) return 1;
     if (force_v > 0) return force_v;
-    if (s->force_tj > 2) return 2;  // tiles of four rows exist for one and two values
-    return four_values_suit(&s->p) ? 4 : 2;
+    if (s->force_tj == 3) return 4;  // tiles of three rows exist for four values
+    if (s->force_tj > 3) return 2;   // tiles of four rows exist for one and two values (eight for one)
+    return four_values_suit(&s->p, s->cus) ? 4 : 2;
+}
+
+// Planes per block of the four-value tiling, whose blocks hold two waves per
+// SIMD (8 per CU): the longest chunk (at most 32 planes) that still gives
+// four rounds of resident waves, so no CU marches much more than another;
+// failing that, the shortest chunk whose waves fit in one round (the most
+// waves that run at once). Measured on MI355X (3-row tiles, two per block,
+// each grid against its other chunks on one device): 1066^2 x 410, 32 planes
+// 1.570 ms (48: 1.591, 64: 1.593, 96: 1.653); 726^2 x 410, 32 (8 and 16
+// within 0.5 %); 486^2 x 410, 16 planes 332 us (8: 337, 32: 358); 326^2 x 410,
+// 8 planes 156 us (16: 166, 32: 188); 206^2 x 410 and 208^2 x 406, 16 planes
+// 67.8 and 69.7 us (8: 71.3 and 72.2, 32: 74.8 and 77.8); 536^2 x 210, 8
+// planes 227 us (16: 232, 32: 259); 366^2 x 210, 4 planes 116 us (8: 117,
+// 16: 128).
+int four_value_chunk(int64_t lateral, int64_t n_windows, int tj, int tpb, int cus) {
+    const int64_t round = (int64_t)cus * 8;
+    for (int c = 32; c >= 4; c /= 2)
+        if (sweep_waves(lateral, n_windows, tj, tpb, c) >= 4 * round) return c;
+    int chunk = 32;
+    for (int c = 32; c >= 4; c /= 2)
+        if (sweep_waves(lateral, n_windows, tj, tpb, c) <= round) chunk = c;
+    return chunk;
 }
 
 // Fixed tiling of one configuration: v values per lane, tj rows per tile,
@@ -318,7 +350,7 @@ SweepShape compute_sweep_shape(const pic_sim *s) {
     const int64_t waves_wanted = (int64_t)cus * (sh.v >= 2 ? 16 : 8);
     int tj = s->force_tj;
     int chunk = s->force_chunk;
-    const int max_tj = (sh.v == 1 && s->prec == PIC_FP32) ? 8 : sh.v == 4 ? 2 : 4;
+    const int max_tj = (sh.v == 1 && s->prec == PIC_FP32) ? 8 : sh.v == 4 ? 3 : 4;
     const int chunk0 = 16;
     if (tj <= 0 && sh.v == 4) tj = max_tj;
     if (tj <= 0) {
@@ -337,15 +369,20 @@ SweepShape compute_sweep_shape(const pic_sim *s) {
                     found = true;
                 }
     }
-    // Four values per lane: two tiles per block marching in lockstep, 32
-    // planes (1066^2 x 410: 1.58 ms per step, against 1.59 for 16 or 64
-    // planes, 1.63 and 1.60 for four tiles of 32 or 64, 1.69 for one tile of
-    // 16 free-running). Smaller grids, with tiles of four rows and two values,
-    // were measured slower in lockstep (208^2 x 406: 75 us alone, 82-97 us in
-    // pairs or fours) and keep one tile per block.
+    // Four values per lane: two tiles per block marching in lockstep (with
+    // 2-row tiles on 1066^2 x 410: 1.58 ms per step, against 1.59 for 16 or
+    // 64 planes, 1.63 and 1.60 for four tiles of 32 or 64, 1.69 for one tile
+    // of 16 free-running), and tiles of three rows (1.570 against 1.588 ms
+    // with 2-row pairs; 208^2 x 406, 69.7 against 74.1 us; three tiles per
+    // block 1.73 ms and 91 us). The register cost of a third row keeps the two
+    // waves per SIMD of 2-row tiles (223-239 against 175-189 VGPRs), and a pair
+    // then reads 8 rows for 6 it writes, against 6 for 4. Tiles of four rows
+    // and two values were measured slower in lockstep (208^2 x 406: 75 us
+    // alone, 82-97 us in pairs or fours) and keep one tile per block.
     const int tpb = s->force_tpb > 0 ? std::min(s->force_tpb, std::max(1, 8 / sh.wpb)) : (sh.v == 4 ? 2 : 1);
+    if (chunk <= 0 && sh.v == 4) chunk = four_value_chunk(rows, sh.n_windows, tj, tpb, cus);
     if (chunk <= 0) {
-        chunk = sh.v == 4 ? 32 : chunk0;
+        chunk = chunk0;
         while (chunk > 4 && ceil_div(rows, tj) * sh.n_windows * ceil_div(planes, chunk) < waves_wanted) chunk /= 2;
     }
     sh.tj = tj;
@@ -371,6 +408,12 @@ const void *sweep_fn_typed(int tj, bool dep, bool reflect) {
     switch (tj) {
         case 1: return sweep_fn_tj<S, A, 1, V>(dep, reflect);
         case 2: return sweep_fn_tj<S, A, 2, V>(dep, reflect);
+        case 3:
+            if constexpr (V == 4) {
+                return sweep_fn_tj<S, A, 3, V>(dep, reflect);
+            } else {
+                return nullptr;
+            }
         case 4:
             if constexpr (V == 4) {
                 return nullptr;
@@ -1754,10 +1797,13 @@ int pic_set_tiling(pic_sim *s, int rows_per_tile, int planes_per_chunk, int valu
         return fail(PIC_E_INVALID, "values_per_lane must be 0, 1, 2 or 4");
     if (values_per_lane >= 2 && s->prec != PIC_FP32)
         return fail(PIC_E_INVALID, "%d values per lane needs FP32 fields", values_per_lane);
-    if (values_per_lane == 4 && rows_per_tile > 2)
-        return fail(PIC_E_INVALID, "four values per lane takes rows_per_tile 0, 1 or 2");
-    if (rows_per_tile && rows_per_tile != 1 && rows_per_tile != 2 && rows_per_tile != 4 && rows_per_tile != 8)
-        return fail(PIC_E_INVALID, "rows_per_tile must be 0, 1, 2, 4 or 8");
+    if (values_per_lane == 4 && rows_per_tile > 3)
+        return fail(PIC_E_INVALID, "four values per lane takes rows_per_tile 0, 1, 2 or 3");
+    if (rows_per_tile && rows_per_tile != 1 && rows_per_tile != 2 && rows_per_tile != 3 && rows_per_tile != 4 &&
+        rows_per_tile != 8)
+        return fail(PIC_E_INVALID, "rows_per_tile must be 0, 1, 2, 3, 4 or 8");
+    if (rows_per_tile == 3 && (s->prec != PIC_FP32 || (values_per_lane != 0 && values_per_lane != 4)))
+        return fail(PIC_E_INVALID, "rows_per_tile 3 needs FP32 fields with four values per lane");
     if (rows_per_tile == 8 && (s->prec != PIC_FP32 || values_per_lane != 1))
         return fail(PIC_E_INVALID, "rows_per_tile 8 needs FP32 fields and values_per_lane 1");
     HIP_TRY(hipStreamSynchronize(s->stream));

@@ -44,7 +44,7 @@ def test_explicit_tiling_and_bad_values():
 
 
 def test_default_tiling_follows_the_grid():
-    """Four values per lane (2-row tiles) only on grids with enough waves for
+    """Four values per lane (3-row tiles) only on grids with enough waves for
     them; automatic values fall back to two under forced 4-row tiles; the
     default four-value run matches two values per lane bit for bit."""
     from configs import readme_config
@@ -61,7 +61,7 @@ def test_default_tiling_follows_the_grid():
     out = []
     with HipSimulation(large, "fp32") as sim:
         tiling = sim.get_tiling()
-        assert (tiling["rows_per_tile"], tiling["values_per_lane"]) == (2, 4)
+        assert (tiling["rows_per_tile"], tiling["values_per_lane"]) == (3, 4)
         sim.set_tiling(4, 16, 0)
         assert sim.get_tiling() == dict(rows_per_tile=4, planes_per_chunk=16, values_per_lane=2)
         sim.set_tiling(0, 0, 0)
