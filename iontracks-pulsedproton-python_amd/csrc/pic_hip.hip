@@ -186,13 +186,16 @@ struct pic_batch {
     size_t ev_used = 0;
     bool graphs = false;       // replay each segment's launches through graph chains (never while timing)
     std::vector<KLaunch> seg;  // one segment's launches
-    // Long-run lane: the configurations that run much longer than the others
-    // (over 0.8 of the longest run, at most a quarter of them) are advanced by
-    // a sub-batch of their own, on its own stream and host thread, beside a
-    // sub-batch of the rest; their latency-bound tail of small launches then
-    // overlaps the rest's sweeps instead of following them. The sub-batches
-    // are built on the first run that splits and kept while the split holds.
-    pic_batch *lanes[2] = {nullptr, nullptr};
+    // Lanes: the configurations that run much longer than the others (over
+    // 0.8 of the longest run, at most a quarter of them) are advanced by a
+    // sub-batch of their own, on its own stream and host thread, beside
+    // sub-batches of the rest (lane_split); their latency-bound tail of small
+    // launches then overlaps the rest's sweeps instead of following them. The
+    // sub-batches are built on the first run that splits and kept while the
+    // split holds.
+    static constexpr int kMaxLanes = 3;
+    pic_batch *lanes[kMaxLanes] = {nullptr, nullptr, nullptr};
+    int n_lanes = 0;
     std::vector<int> lane_of;  // the split the lanes were built for: lane of sims[c]
     bool is_lane = false;      // a sub-batch: never splits again
 };
@@ -384,6 +387,15 @@ SweepShape compute_sweep_shape(const pic_sim *s) {
     if (chunk <= 0) {
         chunk = chunk0;
         while (chunk > 4 && ceil_div(rows, tj) * sh.n_windows * ceil_div(planes, chunk) < waves_wanted) chunk /= 2;
+        // Grids too small for 8 waves per CU with 1-row tiles of 4 planes are
+        // latency-bound: every plane a block marches is one more dependent
+        // memory round, so they take 2-plane chunks (the short march, which
+        // runs no step past its chunk). Measured on MI355X, FP32: 39^2 x 206
+        // 6.8 against 8.15 us per sweep, 21^2 x 166 6.0 against 7.4, 34^2 x
+        // 210 6.3 against 6.7; FP64 39^2 x 206 8.15 against 8.8. 106^2 x 210
+        // (5408 waves at 4 planes) gains 3 %, 166^2 x 210 loses 12 %.
+        if (tj == 1 && chunk == 4 && sh.v <= 2 && ceil_div(rows, 1) * sh.n_windows * ceil_div(planes, 4) < (int64_t)cus * 8)
+            chunk = 2;
     }
     sh.tj = tj;
     sh.n_jt = (int)ceil_div(rows, tj);
@@ -430,8 +442,25 @@ const void *sweep_fn_typed(int tj, bool dep, bool reflect) {
     }
 }
 
-// null for a combination that is not compiled (the tiling checks reject it first)
-const void *sweep_fn(int prec, int v, int tj, bool dep, bool reflect) {
+// The depositing kernel of 1-row tiles marching exactly `ch` (1 or 2) planes.
+template <typename S, typename A, int V>
+const void *sweep_fn_short(int ch, bool reflect) {
+    if (ch == 1)
+        return reflect ? (const void *)pic::sweep_kernel<S, A, 1, true, V, true, 1>
+                       : (const void *)pic::sweep_kernel<S, A, 1, true, V, false, 1>;
+    return reflect ? (const void *)pic::sweep_kernel<S, A, 1, true, V, true, 2>
+                   : (const void *)pic::sweep_kernel<S, A, 1, true, V, false, 2>;
+}
+
+// null for a combination that is not compiled (the tiling checks reject it
+// first). Depositing 1-row tiles of one or two planes take the short march,
+// which runs no step past the chunk (the general march runs its plane
+// buffers' count); the arithmetic is the same.
+const void *sweep_fn(int prec, int v, int tj, bool dep, bool reflect, int chunk = 0) {
+    if (tj == 1 && dep && chunk >= 1 && chunk <= 2 && v <= 2) {
+        if (prec == PIC_FP64) return sweep_fn_short<double, double, 1>(chunk, reflect);
+        return v == 2 ? sweep_fn_short<float, float, 2>(chunk, reflect) : sweep_fn_short<float, float, 1>(chunk, reflect);
+    }
     if (prec == PIC_FP64) return sweep_fn_typed<double, double, 1>(tj, dep, reflect);
     if (v == 4) return sweep_fn_typed<float, float, 4>(tj, dep, reflect);
     return v == 2 ? sweep_fn_typed<float, float, 2>(tj, dep, reflect) : sweep_fn_typed<float, float, 1>(tj, dep, reflect);
@@ -455,7 +484,18 @@ const void *batch_fn_typed(int tj, bool reflect) {
     }
 }
 
-const void *batch_fn(int prec, int v, int tj, bool reflect) {
+const void *batch_fn(int prec, int v, int tj, bool reflect, int chunk = 0) {
+    if (tj == 1 && chunk >= 1 && chunk <= 2 && (prec == PIC_FP64 || v == 2)) {
+        if (prec == PIC_FP64)
+            return chunk == 1 ? (reflect ? (const void *)pic::batch_sweep_kernel<double, double, 1, 1, true, 1>
+                                         : (const void *)pic::batch_sweep_kernel<double, double, 1, 1, false, 1>)
+                              : (reflect ? (const void *)pic::batch_sweep_kernel<double, double, 1, 1, true, 2>
+                                         : (const void *)pic::batch_sweep_kernel<double, double, 1, 1, false, 2>);
+        return chunk == 1 ? (reflect ? (const void *)pic::batch_sweep_kernel<float, float, 1, 2, true, 1>
+                                     : (const void *)pic::batch_sweep_kernel<float, float, 1, 2, false, 1>)
+                          : (reflect ? (const void *)pic::batch_sweep_kernel<float, float, 1, 2, true, 2>
+                                     : (const void *)pic::batch_sweep_kernel<float, float, 1, 2, false, 2>);
+    }
     if (prec == PIC_FP64) return batch_fn_typed<double, double, 1>(tj, reflect);
     return v == 2 ? batch_fn_typed<float, float, 2>(tj, reflect) : batch_fn_typed<float, float, 1>(tj, reflect);
 }
@@ -675,7 +715,7 @@ int emit(pic_sim *s, const KLaunch &l, hipStream_t st) {
 }
 
 const void *checked_sweep_fn(const pic_sim *s, const SweepShape &sh, bool dep) {
-    const void *f = sweep_fn(s->prec, sh.v, sh.tj, dep, s->p.wall == PIC_WALL_REFLECTING);
+    const void *f = sweep_fn(s->prec, sh.v, sh.tj, dep, s->p.wall == PIC_WALL_REFLECTING, sh.chunk);
     if (!f) fail(PIC_E_INVALID, "no sweep kernel for %d rows per tile with %d values per lane (%s fields)", sh.tj, sh.v,
                  s->prec == PIC_FP64 ? "FP64" : "FP32");
     return f;
@@ -985,8 +1025,9 @@ size_t partial_cap_bytes(const pic_params *p) {
 // configurations running at that step at least waves_min waves. Measured on
 // MI355X over a 32-point saturation curve (16^2..47^2 x 106..206 grids):
 // 2-row tiles of 4 planes are fastest while eight or more grids run, 1-row
-// tiles of 4 planes once a few are left; taller tiles lose to the register
-// cost of the multi-configuration kernel, longer chunks to the march length.
+// tiles of 4 planes once a few are left, and 1-row tiles of 2 planes (the
+// short march) for the last few; taller tiles lose to the register cost of
+// the multi-configuration kernel, longer chunks to the march length.
 int batch_tiling(pic_batch *b) {
     int v = 2, wpb = 8;
     for (pic_sim *s : b->sims) v = std::min(v, values_per_lane(s, 0));
@@ -997,7 +1038,7 @@ int batch_tiling(pic_batch *b) {
     if (b->force_tj > 0 || b->force_chunk > 0)
         cands = {{b->force_tj > 0 ? b->force_tj : 1, b->force_chunk > 0 ? b->force_chunk : 4}};
     else
-        cands = {{2, 4}, {1, 4}};
+        cands = {{2, 4}, {1, 4}, {1, 2}};
     std::vector<std::vector<SweepShape>> shapes(cands.size());
     for (size_t ci = 0; ci < cands.size(); ++ci)
         for (pic_sim *s : b->sims) shapes[ci].push_back(fixed_shape(s, v, cands[ci].first, cands[ci].second, wpb));
@@ -1059,38 +1100,54 @@ int ensure_batch_slots(pic_batch *b, int64_t steps) {
 
 // --- batch lanes -----------------------------------------------------
 
-// The long-run lane of a batch: part[c] = 0 for the configurations running
-// over 0.8 of the longest run, 1 for the others. A split needs at least four
-// configurations and puts at most a quarter of them in the long lane; false
-// when there is no such split. (Measured on the 32-point saturation curve:
-// the longest one to three runs in a lane of their own take the batch from
-// 38.9 to 34.7-35.3 ms; four to twelve, 36.3-37.1 ms; three lanes, 44 ms.)
-bool lane_split(const pic_batch *b, std::vector<int> &part) {
+// The lanes of a batch, by run length: part[c] = 0 for the configurations
+// running over 0.8 of the longest run, 1 for those over 0.36 of it, 2 for the
+// rest. The long lane needs at least four configurations in all and takes at
+// most a quarter of them; the middle lane is kept only when it and the last
+// lane both hold two configurations or more, else it joins the last. Returns
+// the number of lanes, 0 when there is no split. Measured on the 32-point
+// saturation curve, lanes driven from Python threads with the split switched
+// off, median of five passes: one lane 36.9 ms; the longest run apart 33.1;
+// the longest (2505 steps) | the eight of 927-1993 steps | the 23 of at most
+// 883, 31.1 (16.5, 20.5 and 15.0 ms alone); four lanes balanced by a cost
+// model 32.3; six or eight lanes 44.7-45.4 (more streams than the four
+// hardware queues of a process).
+int lane_split(const pic_batch *b, std::vector<int> &part) {
     const int n = (int)b->sims.size();
-    if (n < 4) return false;
+    if (n < 4) return 0;
     int64_t longest = 0;
     for (const pic_sim *s : b->sims) longest = std::max(longest, s->n_steps);
-    part.assign(n, 1);
-    int n_long = 0;
-    for (int c = 0; c < n; ++c)
-        if ((double)b->sims[c]->n_steps > 0.8 * (double)longest) {
-            part[c] = 0;
-            ++n_long;
-        }
-    return n_long >= 1 && n_long <= n / 4;
+    part.assign(n, 2);
+    int count[3] = {0, 0, 0};
+    for (int c = 0; c < n; ++c) {
+        const double r = (double)b->sims[c]->n_steps / (double)longest;
+        part[c] = r > 0.8 ? 0 : r > 0.36 ? 1 : 2;
+        ++count[part[c]];
+    }
+    if (count[0] < 1 || count[0] > n / 4) return 0;
+    if (count[1] < 2 || count[2] < 2) {  // two lanes: the long one and the rest
+        for (int c = 0; c < n; ++c)
+            if (part[c] == 2) part[c] = 1;
+        return 2;
+    }
+    return 3;
 }
 
-// Run steps [first, first+count) as two lanes (sub-batches over the split
-// `part`), each driven by its own host thread on its own stream; the batch's
-// stream then waits for both, and each handle's stream for its lane.
-int run_lanes(pic_batch *b, const std::vector<int> &part, int64_t first, int64_t count) {
-    if (part != b->lane_of || !b->lanes[0] || !b->lanes[1]) {
+// Run steps [first, first+count) as lanes (sub-batches over the split `part`
+// of n_lanes lanes), each driven by its own host thread on its own stream; the
+// batch's stream then waits for all of them, and each handle's stream for its
+// lane.
+int run_lanes(pic_batch *b, const std::vector<int> &part, int n_lanes, int64_t first, int64_t count) {
+    bool built = part == b->lane_of && n_lanes == b->n_lanes;
+    for (int L = 0; L < n_lanes && built; ++L) built = b->lanes[L] != nullptr;
+    if (!built) {
         for (pic_batch *&l : b->lanes) {
             pic_batch_destroy(l);
             l = nullptr;
         }
         b->lane_of.clear();
-        for (int L = 0; L < 2; ++L) {
+        b->n_lanes = 0;
+        for (int L = 0; L < n_lanes; ++L) {
             std::vector<pic_sim *> members;
             for (size_t c = 0; c < b->sims.size(); ++c)
                 if (part[c] == L) members.push_back(b->sims[c]);
@@ -1102,9 +1159,10 @@ int run_lanes(pic_batch *b, const std::vector<int> &part, int64_t first, int64_t
             b->lanes[L]->graphs = b->graphs;
         }
         b->lane_of = part;
+        b->n_lanes = n_lanes;
     }
-    int rc[2] = {PIC_OK, PIC_OK};
-    std::string err[2];
+    int rc[pic_batch::kMaxLanes] = {PIC_OK, PIC_OK, PIC_OK};
+    std::string err[pic_batch::kMaxLanes];
     auto go = [&](int L) {
         pic_batch *l = b->lanes[L];
         int64_t longest = 0;
@@ -1113,30 +1171,35 @@ int run_lanes(pic_batch *b, const std::vector<int> &part, int64_t first, int64_t
         if (c > 0) rc[L] = pic_batch_run(l, first, c);
         if (rc[L]) err[L] = g_err;
     };
+    // lanes 0 .. n_lanes-2 on worker threads, the last on this one
     const int device = b->sims[0]->device;
-    std::thread worker;
-    bool threaded = true;
-    try {
-        worker = std::thread([&] {
-            if (hipSetDevice(device) != hipSuccess) {
-                rc[0] = fail(PIC_E_HIP, "hipSetDevice failed in the lane thread");
-                err[0] = g_err;
-                return;
-            }
-            go(0);
-        });
-    } catch (...) {  // no thread to be had: both lanes from this one, one after the other
-        threaded = false;
+    std::thread workers[pic_batch::kMaxLanes];
+    bool threaded[pic_batch::kMaxLanes] = {false, false, false};
+    for (int L = 0; L + 1 < n_lanes; ++L) {
+        try {
+            workers[L] = std::thread([&, L] {
+                if (hipSetDevice(device) != hipSuccess) {
+                    rc[L] = fail(PIC_E_HIP, "hipSetDevice failed in a lane thread");
+                    err[L] = g_err;
+                    return;
+                }
+                go(L);
+            });
+            threaded[L] = true;
+        } catch (...) {  // no thread to be had: that lane from this one, after the last
+        }
     }
-    go(1);
-    if (threaded) worker.join();
-    else go(0);
-    for (int L = 0; L < 2; ++L)
+    go(n_lanes - 1);
+    for (int L = 0; L + 1 < n_lanes; ++L) {
+        if (threaded[L]) workers[L].join();
+        else go(L);
+    }
+    for (int L = 0; L < n_lanes; ++L)
         if (rc[L]) {
             snprintf(g_err, sizeof g_err, "%s", err[L].c_str());
             return rc[L];
         }
-    for (pic_batch *l : b->lanes) HIP_TRY(hipStreamWaitEvent(b->stream, l->done, 0));
+    for (int L = 0; L < n_lanes; ++L) HIP_TRY(hipStreamWaitEvent(b->stream, b->lanes[L]->done, 0));
     return PIC_OK;
 }
 
@@ -1967,6 +2030,7 @@ int pic_batch_set_tiling(pic_batch *b, int rows_per_tile, int planes_per_chunk) 
         l = nullptr;
     }
     b->lane_of.clear();
+    b->n_lanes = 0;
     const int old_tj = b->force_tj, old_chunk = b->force_chunk;
     b->force_tj = rows_per_tile;
     b->force_chunk = planes_per_chunk;
@@ -2000,7 +2064,8 @@ int pic_batch_run(pic_batch *b, int64_t first, int64_t count) {
     // lanes drive launches from two host threads: not while timing
     if (!b->is_lane && !b->timing) {
         std::vector<int> part;
-        if (lane_split(b, part)) return run_lanes(b, part, first, count);
+        const int n_lanes = lane_split(b, part);
+        if (n_lanes > 1) return run_lanes(b, part, n_lanes, first, count);
     }
     // run order: longest first, so the configurations still running at any
     // step are a prefix and the grid covers exactly their blocks
@@ -2107,7 +2172,7 @@ int pic_batch_run(pic_batch *b, int64_t first, int64_t count) {
             const dim3 grid((unsigned)(o[na] + na));  // na fold blocks, then the sweep blocks
             const auto table = batch_table(b->descs_d[k] + (t - s0) * n, b->offs_d[k] + (t - s0) * (n + 1), na);
             const int tj = b->cands[cand_of[t - s0]].first;
-            const void *f = batch_fn(b->prec, b->v, tj, b->reflect);
+            const void *f = batch_fn(b->prec, b->v, tj, b->reflect, b->cands[cand_of[t - s0]].second);
             if (!f) rc = fail(PIC_E_INVALID, "no batched sweep kernel for %d rows per tile", tj);
             else b->seg.push_back(KLaunch::make(f, grid, block, 0, table));
         }

@@ -284,8 +284,11 @@ constexpr int kBufferWord3 = 0x00020000;
 //
 // sweep_body runs one block's share: `logical` is the block's index among the
 // `nblk` blocks of this configuration (already XCD-remapped), DEP says whether
-// this step deposits (a runtime flag in the multi-configuration kernel).
-template <typename S, typename A, int TJ, int V, bool REFLECT>
+// this step deposits (a runtime flag in the multi-configuration kernel). CH > 0
+// compiles a march of exactly CH planes (chunks of at most CH planes): no
+// step past the chunk runs, and the last step loads nothing ahead. CH = 0
+// marches any chunk in groups of the plane-buffer count.
+template <typename S, typename A, int TJ, int V, bool REFLECT, int CH = 0>
 __device__ __forceinline__ void sweep_body(const SweepArgs &a, const int logical, const bool DEP) {
     static_assert(V == 1 || ((V == 2 || V == 4) && sizeof(S) == 4), "two or four values per lane are FP32 layouts");
     constexpr int WIN = window_outputs(V);  // k step between windows (V = 2: a.win_step, 124 or 126)
@@ -575,7 +578,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &a, const int logical
             // loads a repeat of plane i1, computes, and stores nothing
             const bool live = i < i1;
             if (lockstep) __syncthreads();
-            load_rows(min(i + 1 + D, i1), bp[LB], bn[LB], fd[LB]);
+            if constexpr (CH == 0 || O + 1 < CH) load_rows(min(i + 1 + D, i1), bp[LB], bn[LB], fd[LB]);
             add_dep_rows(bp[XB], bn[XB], fd[XB]);
             if (!REFLECT && live) {
                 carry_ring_rows(i, bp[CB], bn[CB], false);
@@ -677,8 +680,13 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &a, const int logical
                 }
             }
         };
-        for (int i = i0; i < i1; i += NB) {
-            static_for<NB>([&](auto Oc) { plane_step(i + decltype(Oc)::value, Oc); });
+        if constexpr (CH > 0) {
+            static_assert(CH <= NB, "a short march is one group of plane buffers");
+            static_for<CH>([&](auto Oc) { plane_step(i0 + decltype(Oc)::value, Oc); });
+        } else {
+            for (int i = i0; i < i1; i += NB) {
+                static_for<NB>([&](auto Oc) { plane_step(i + decltype(Oc)::value, Oc); });
+            }
         }
         if constexpr (kMerged) {
             acc_r += sr[0];
@@ -714,14 +722,14 @@ __device__ __forceinline__ void fold_previous(const SweepArgs &a) {
 // One configuration, one step. Block 0 folds the previous step when one is
 // pending (prev_out4 set), the other blocks sweep. Shifting the sweep blocks
 // by one keeps each XCD's run of logical tiles on a single XCD.
-template <typename S, typename A, int TJ, bool DEP, int V, bool REFLECT>
+template <typename S, typename A, int TJ, bool DEP, int V, bool REFLECT, int CH = 0>
 __global__ void __launch_bounds__(512) sweep_kernel(SweepArgs a) {
     const int nf = a.prev_out4 ? 1 : 0;
     if ((int)blockIdx.x < nf) {
         fold_previous(a);
         return;
     }
-    sweep_body<S, A, TJ, V, REFLECT>(a, xcd_logical_block(blockIdx.x - nf, gridDim.x - nf), DEP);
+    sweep_body<S, A, TJ, V, REFLECT, CH>(a, xcd_logical_block(blockIdx.x - nf, gridDim.x - nf), DEP);
 }
 
 // The one argument of a batched launch: per-configuration descriptors, the
@@ -761,7 +769,7 @@ __device__ __forceinline__ void find_config(const BatchTable<D> &t, int g, int &
 // single-configuration kernel): one that deposits nothing this step points
 // dens at zeros, and x + 0.0 == x. Configurations are ordered so the active
 // ones are a prefix; the grid covers exactly those.
-template <typename S, typename A, int TJ, int V, bool REFLECT>
+template <typename S, typename A, int TJ, int V, bool REFLECT, int CH = 0>
 __global__ void __launch_bounds__(512) batch_sweep_kernel(BatchTable<SweepArgs> t) {
     const SweepArgs *__restrict__ descs = t.descs;
     const int n_cfg = t.n_cfg;
@@ -774,7 +782,7 @@ __global__ void __launch_bounds__(512) batch_sweep_kernel(BatchTable<SweepArgs> 
     int c, first;
     find_config(t, g, c, first);
     const SweepArgs a = descs[c];
-    sweep_body<S, A, TJ, V, REFLECT>(a, g - first, true);
+    sweep_body<S, A, TJ, V, REFLECT, CH>(a, g - first, true);
 }
 
 // Tracks binned by gather tile, for one launch's steps (built on the device
