@@ -113,8 +113,10 @@ def test_reflecting_two_species_batch_matches_the_oracle():
         assert res.ks == pytest.approx(ref.ks, rel=RTOL)
 
 
-@pytest.mark.parametrize("tiling", [(1, 4), (2, 8), (4, 16)])
+@pytest.mark.parametrize("tiling", [(1, 4), (2, 8), (4, 16), (1, 2), (1, 1)])
 def test_every_batch_tiling_gives_the_same_fields(tiling):
+    """(1, 2) and (1, 1) run the short march (exactly one or two planes per
+    block), the others the general one."""
     cfgs = uneven_configs()[:3]
     base = run_batch_hip(cfgs, precision="fp32", return_fields=True)
     got = run_batch_hip(cfgs, precision="fp32", return_fields=True, tiling=tiling)
@@ -122,6 +124,27 @@ def test_every_batch_tiling_gives_the_same_fields(tiling):
         np.testing.assert_array_equal(a.positive_array, b.positive_array)
         np.testing.assert_array_equal(a.negative_array, b.negative_array)
         assert a.ks == pytest.approx(b.ks, rel=RTOL)
+
+
+def three_lane_configs():
+    """Eight small curve points whose run lengths split a batch into three
+    lanes: one run over 0.8 of the longest (1133 steps), three over 0.36 of
+    it (432-696), four shorter (189-337); rows of one and two k windows."""
+    pts = [(12.5, 0.0077, 258.97), (12.5, 0.0101, 134.13), (16.0, 0.0077, 134.13), (16.0, 0.0052, 96.53),
+           (20.0, 0.0101, 96.53), (16.0, 0.0077, 50.0), (20.0, 0.0077, 50.0), (25.0, 0.0052, 50.0)]
+    return [readme_config(g, r, voltage_V=v, dose_rate_Gy_s=8.91) for g, r, v in pts]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_a_batch_in_three_lanes_matches_each_configuration_run_alone(precision):
+    cfgs = three_lane_configs()
+    steps = np.array([c.total_time_steps for c in cfgs])
+    ratio = steps / steps.max()
+    # the library's split: over 0.8 of the longest, over 0.36, the rest
+    assert (ratio > 0.8).sum() == 1 and ((ratio > 0.36) & (ratio <= 0.8)).sum() == 3 and (ratio <= 0.36).sum() == 4
+    got = run_batch_hip(cfgs, precision=precision, return_fields=True)
+    for cfg, res in zip(cfgs, got):
+        _check_same(res, run_simulation_hip(cfg, progress=False, precision=precision))
 
 
 def test_a_batch_run_in_pieces_equals_one_run():

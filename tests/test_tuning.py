@@ -43,6 +43,22 @@ def test_explicit_tiling_and_bad_values():
         run_simulation_hip(cfg, progress=False, tiling=(3, 4, 1))
 
 
+@pytest.mark.parametrize("precision, values", [("fp32", 2), ("fp32", 1), ("fp64", 0)])
+@pytest.mark.parametrize("wall", ["absorbing", "reflecting"])
+def test_short_march_matches_the_general_march(precision, values, wall):
+    """1-row tiles of one or two planes run the short march (exactly that many
+    plane steps); four planes the general one. Same fields bit for bit."""
+    from configs import readme_config
+
+    cfg = readme_config(20.0, 0.0101, voltage_V=96.53, dose_rate_Gy_s=40.0, lateral_boundary=wall)
+    runs = [run_simulation_hip(cfg, progress=False, precision=precision, tiling=(1, chunk, values)) for chunk in (4, 2, 1)]
+    for other in runs[1:]:
+        np.testing.assert_array_equal(other.positive_array, runs[0].positive_array)
+        np.testing.assert_array_equal(other.negative_array, runs[0].negative_array)
+        assert other.ks == pytest.approx(runs[0].ks, rel=1e-12)
+    assert runs[0].injected_positive.sum() > 0
+
+
 def test_default_tiling_follows_the_grid():
     """Four values per lane (3-row tiles) only on grids with enough waves for
     them; automatic values fall back to two under forced 4-row tiles; the
