@@ -164,9 +164,10 @@ int pic_get_gather_mode(pic_sim *sim, int *binned);
 int pic_synchronize(pic_sim *sim);
 
 /* Sweep launch shape, for tuning: rows per tile (1, 2, 3, 4, 8), planes per
- * block chunk, and k values per lane (1; 2 or 4 for FP32 fields: four with
- * rows per tile 1, 2 or 3 only, three rows with four values only, eight rows
- * with one value only); 0 = automatic. Takes effect at the next launch. */
+ * block chunk, and k values per lane (1 or 2 for FP64 fields, two with rows
+ * per tile 1 or 2 only; 1, 2 or 4 for FP32 fields, four with rows per tile 1,
+ * 2 or 3 only; three rows with four FP32 values only, eight rows with one
+ * FP32 value only); 0 = automatic. Takes effect at the next launch. */
 int pic_set_tiling(pic_sim *sim, int rows_per_tile, int planes_per_chunk, int values_per_lane);
 
 /* Consecutive j tiles per sweep block (0 = automatic, at most 8 waves per

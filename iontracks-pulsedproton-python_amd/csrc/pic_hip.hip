@@ -234,35 +234,83 @@ int sm_count(int device) {
     return v;
 }
 
+// Windows (waves) per tile in a block of `tpb` tiles. A block holds every
+// window of its rows when they fit in 8 waves (the sweep kernels' 512-thread
+// bound): a row is then read and written whole by one CU. Longer rows are
+// split into equal groups where the window count allows: 4 windows when it
+// divides by 4 (one- and two-value layouts only), else 2 when it is even, 3
+// when it divides by 3, else 2 (the last group short). A short group is what
+// costs: its blocks hold a full block's place on a CU with some waves idle.
+// Measured on MI355X, sweep times on one device. Two FP64 values, 2-row
+// lockstep pairs: 208^2 x 506 (5 windows), groups of 2 179 us (1: 181,
+// 3+2: 215, 4+1: 243; all 5 in one-tile blocks: 190); 408^2 x 631 (6), 2
+// 0.777 ms (3: 0.807, 1: 0.888, all 6 in one-tile blocks: 0.808); 248^2 x 806
+// (7), 2 388 us (1: 386, 3: 399, 4: 404, all 7 in one-tile blocks: 399);
+// 506^2 x 1010 (9), 3 1.82 ms (2: 1.89, 1 and 4: 1.94, 8+1 in one-tile
+// blocks: 3.04). Two FP32 values, 4-row tiles: 506^2 x 1010 (8 windows), two
+// groups of 4 in lockstep pairs 0.97 ms (2: 0.97; all 8 in one-tile blocks:
+// 1.04); 2656^2 x 1010 (8), 25.6 ms (2: 26.1; all 8: 27.1); 648^2 x 806 (7),
+// all 7 in one-tile blocks 1.31 ms (pairs of tiles, groups of 1: 1.36, 2:
+// 1.39, 4: 1.49); 408^2 x 631 (5), all 5 0.448 ms (groups of 2 and 1 in pairs
+// of tiles: 0.441 and 0.442; 4: 0.641).
+int window_group(int n_windows, int tpb, bool wide) {
+    const int cap = std::max(1, 8 / tpb);
+    if (n_windows <= cap) return n_windows;
+    if (!wide && cap >= 4 && n_windows % 4 == 0) return 4;
+    for (int d = 2; d <= std::min(cap, 3); ++d)
+        if (n_windows % d == 0) return d;
+    return std::min(cap, 2);
+}
+
 // Waves of a sweep tiling: blocks of `tpb` tiles of `tj` rows times `wpb`
 // windows, over every (window group, tile group, chunk of `chunk` planes).
-int64_t sweep_waves(int64_t lateral, int64_t n_windows, int tj, int tpb, int chunk) {
-    const int64_t wpb = std::min<int64_t>(n_windows, 8);
+int64_t sweep_waves(int64_t lateral, int64_t n_windows, int wpb, int tj, int tpb, int chunk) {
     return ceil_div(n_windows, wpb) * ceil_div(ceil_div(lateral, tj), tpb) * ceil_div(lateral, chunk) * wpb * tpb;
 }
 
-// k values per lane: FP64 fields one (8-byte loads); FP32 fields one, two or
-// four (4-, 8- or 16-byte loads). FP32 grids take four when four-value
-// windows take no more lanes than two-value ones and the four-value tiling
-// (3-row tiles, two per block) gives at least 6 waves per CU at 16-plane
-// chunks; two otherwise. Measured on MI355X, each pair on one device: 206^2
-// x 410 and 208^2 x 406, four values (3-row pairs) 67.8 and 69.7 us against
-// 72.9 and 74.3 for the two-value tiling (4-row tiles of 8 planes); 166^2 and
-// 246^2 x 210 (one window), two values 27.1 and 50.8 us against 28.6 and
-// 52.0; 2656^2 x 1010 (nine two-value windows, five four-value ones), 26.9
-// against 30.2 ms; the 16^2..47^2 x 106..206 grids of a saturation curve run
-// one after another, 84 against 108 ms. Grids that took four values with
-// 2-row tiles are 1-9 % faster with 3-row ones (compute_sweep_shape).
+// k values per lane: FP64 fields one or two (8- or 16-byte loads); FP32
+// fields one, two or four (4-, 8- or 16-byte loads). FP32 grids take four when
+// four-value windows take no more lanes than two-value ones and the four-value
+// tiling (3-row tiles, two per block) gives at least 6 waves per CU at
+// 16-plane chunks; two otherwise. Measured on MI355X, each pair on one device:
+// 206^2 x 410 and 208^2 x 406, four values (3-row pairs) 67.8 and 69.7 us
+// against 72.9 and 74.3 for the two-value tiling (4-row tiles of 8 planes);
+// 166^2 and 246^2 x 210 (one window), two values 27.1 and 50.8 us against
+// 28.6 and 52.0; 2656^2 x 1010 (nine two-value windows, five four-value
+// ones), 26.9 against 30.2 ms; the 16^2..47^2 x 106..206 grids of a
+// saturation curve run one after another, 84 against 108 ms. Grids that took
+// four values with 2-row tiles are 1-9 % faster with 3-row ones
+// (compute_sweep_shape).
 bool four_values_suit(const pic_params *p, int cus) {
     const int64_t n = p->nzb - 2, w4 = pic::window_outputs(4), w2 = pic::window_step(2, n);
     const int64_t lateral = p->no_xy - 2;
     const bool lanes_ok = ceil_div(n, w4) * w4 <= ceil_div(n, w2) * w2;
-    const bool waves_ok = sweep_waves(lateral, ceil_div(n, w4), 3, 2, 16) >= (int64_t)cus * 6;
+    const int nw = (int)ceil_div(n, w4);
+    const bool waves_ok = sweep_waves(lateral, nw, window_group(nw, 2, true), 3, 2, 16) >= (int64_t)cus * 6;
     return lanes_ok && waves_ok;
 }
 
+// FP64 grids take two values per lane (2-row tiles in lockstep pairs) when
+// that tiling gives at least 4 waves per CU at 4-plane chunks, whatever the
+// lanes the wider windows leave idle; one value per lane otherwise. Measured
+// on MI355X, sweep time against the one-value tiling, same device: 1066^2 x
+// 410, 3.07 against 3.67 ms; 208^2 x 406, 125 against 155 us; 536^2 x 210,
+// 411 against 487 us; 506^2 x 1010, 1.82 against 2.11 ms; 246^2, 166^2 and
+// 106^2 x 210, 92, 37 and 20 against 109, 39 and 22 us; 128^2, 88^2 and 48^2
+// x 406, 38, 23 and 13 against 44, 35 and 15 us; 46^2 x 210 (484 waves), 9.5
+// against 8.5 us.
+bool fp64_pairs_suit(const pic_params *p, int cus) {
+    const int64_t lateral = p->no_xy - 2;
+    const int nw = (int)ceil_div(p->nzb - 2, pic::window_outputs(2));
+    return sweep_waves(lateral, nw, window_group(nw, 2, true), 2, 2, 4) >= (int64_t)cus * 4;
+}
+
 int values_per_lane(const pic_sim *s, int force_v) {
-    if (s->prec == PIC_FP64) return 1;
+    if (s->prec == PIC_FP64) {
+        if (force_v > 0) return force_v;
+        if (s->force_tj > 2) return 1;  // two FP64 values per lane take 1- and 2-row tiles
+        return fp64_pairs_suit(&s->p, s->cus) ? 2 : 1;
+    }
     if (force_v > 0) return force_v;
     if (s->force_tj == 3) return 4;  // tiles of three rows exist for four values
     if (s->force_tj > 3) return 2;   // tiles of four rows exist for one and two values (eight for one)
@@ -281,14 +329,38 @@ int values_per_lane(const pic_sim *s, int force_v) {
 // 67.8 and 69.7 us (8: 71.3 and 72.2, 32: 74.8 and 77.8); 536^2 x 210, 8
 // planes 227 us (16: 232, 32: 259); 366^2 x 210, 4 planes 116 us (8: 117,
 // 16: 128).
-int four_value_chunk(int64_t lateral, int64_t n_windows, int tj, int tpb, int cus) {
+int four_value_chunk(int64_t lateral, int64_t n_windows, int wpb, int tj, int tpb, int cus) {
     const int64_t round = (int64_t)cus * 8;
     for (int c = 32; c >= 4; c /= 2)
-        if (sweep_waves(lateral, n_windows, tj, tpb, c) >= 4 * round) return c;
+        if (sweep_waves(lateral, n_windows, wpb, tj, tpb, c) >= 4 * round) return c;
     int chunk = 32;
     for (int c = 32; c >= 4; c /= 2)
-        if (sweep_waves(lateral, n_windows, tj, tpb, c) <= round) chunk = c;
+        if (sweep_waves(lateral, n_windows, wpb, tj, tpb, c) <= round) chunk = c;
     return chunk;
+}
+
+// Planes per block of two-FP64-value tiles: the chunk whose busiest CU
+// marches the fewest planes, counted as rounds of resident blocks (8 waves
+// per CU) times chunk + 2 planes (a march loads two planes before its first
+// output); a shorter chunk is taken only when that count is 2 % lower. The
+// count ranks the measured chunks of every grid tried within 1-3 %: on
+// MI355X, 208^2 x 406, 24 planes 124 us (16: 125, 12: 129, 8: 135, 32: 147);
+// 128^2 x 406, 16 planes 38 us (8: 41, 4: 49, 32: 56); 88^2 x 406, 8 planes
+// 23 us (4: 28, 16: 32); 246^2 x 210, 32 planes 92 us (4-16: 92-95); 1066^2
+// x 410, 32 planes 3.07 ms (16: 3.11; 48 and 64, outside the list: 3.03).
+int pair_chunk(const SweepShape &sh, int64_t planes, int cus) {
+    const int64_t slots = (int64_t)cus * std::max(1, 8 / (sh.wpb * sh.tpb));
+    const int64_t groups = (int64_t)sh.n_kg * ceil_div(sh.n_jt, sh.tpb);
+    int best = 32;
+    double best_cost = 0;
+    for (int c : {32, 24, 16, 12, 8, 4}) {
+        const double cost = (double)ceil_div(groups * ceil_div(planes, c), slots) * (c + 2);
+        if (c == 32 || cost < 0.98 * best_cost) {
+            best = c;
+            best_cost = cost;
+        }
+    }
+    return best;
 }
 
 // Fixed tiling of one configuration: v values per lane, tj rows per tile,
@@ -325,9 +397,18 @@ SweepShape compute_sweep_shape(const pic_sim *s) {
     SweepShape sh;
     const int64_t no_xy = s->p.no_xy, nzb = s->p.nzb;
     sh.v = values_per_lane(s, s->force_v);
-    sh.win_step = pic::window_step(sh.v, nzb - 2);
+    // FP64 windows of two values step by 124 (the 126 step of FP32 pairs
+    // needs single-value edge stores)
+    sh.win_step = s->prec == PIC_FP64 ? pic::window_outputs(sh.v) : pic::window_step(sh.v, nzb - 2);
     sh.n_windows = (int)ceil_div(nzb - 2, sh.win_step);
-    sh.wpb = std::min(sh.n_windows, 8);
+    // 16-byte lanes: four FP32 or two FP64 values
+    const bool wide = sh.v == 4 || (s->prec == PIC_FP64 && sh.v == 2);
+    // Tiles per block: two in lockstep for 16-byte lanes and for rows of 8 or
+    // more windows (window_group); one otherwise. A forced count is cut to
+    // what a block of every window of a row holds (to 4 on longer rows).
+    const int fit = std::max(1, 8 / std::min(sh.n_windows, 8));
+    sh.tpb = s->force_tpb > 0 ? std::min(s->force_tpb, sh.n_windows > 8 ? 4 : fit) : (wide || sh.n_windows >= 8 ? 2 : 1);
+    sh.wpb = window_group(sh.n_windows, sh.tpb, wide);
     sh.n_kg = (int)ceil_div(sh.n_windows, sh.wpb);
     const int64_t rows = no_xy - 2, planes = no_xy - 2;
     const int cus = s->cus;
@@ -353,9 +434,11 @@ SweepShape compute_sweep_shape(const pic_sim *s) {
     const int64_t waves_wanted = (int64_t)cus * (sh.v >= 2 ? 16 : 8);
     int tj = s->force_tj;
     int chunk = s->force_chunk;
-    const int max_tj = (sh.v == 1 && s->prec == PIC_FP32) ? 8 : sh.v == 4 ? 3 : 4;
+    // tallest tiles: three rows of four FP32 values, two of two FP64 values
+    // (a third row spills registers in the absorbing-wall kernel)
+    const int max_tj = (sh.v == 1 && s->prec == PIC_FP32) ? 8 : sh.v == 4 ? 3 : wide ? 2 : 4;
     const int chunk0 = 16;
-    if (tj <= 0 && sh.v == 4) tj = max_tj;
+    if (tj <= 0 && wide) tj = max_tj;
     if (tj <= 0) {
         // the tallest tile that reaches the wave count, with two values per
         // lane at chunks of 16 or, failing that, 8 planes before a shorter
@@ -381,9 +464,12 @@ SweepShape compute_sweep_shape(const pic_sim *s) {
     // waves per SIMD of 2-row tiles (223-239 against 175-189 VGPRs), and a pair
     // then reads 8 rows for 6 it writes, against 6 for 4. Tiles of four rows
     // and two values were measured slower in lockstep (208^2 x 406: 75 us
-    // alone, 82-97 us in pairs or fours) and keep one tile per block.
-    const int tpb = s->force_tpb > 0 ? std::min(s->force_tpb, std::max(1, 8 / sh.wpb)) : (sh.v == 4 ? 2 : 1);
-    if (chunk <= 0 && sh.v == 4) chunk = four_value_chunk(rows, sh.n_windows, tj, tpb, cus);
+    // alone, 82-97 us in pairs or fours) and keep one tile per block on rows
+    // of up to 7 windows (window_group covers longer rows).
+    sh.tj = tj;
+    sh.n_jt = (int)ceil_div(rows, tj);
+    if (chunk <= 0 && sh.v == 4) chunk = four_value_chunk(rows, sh.n_windows, sh.wpb, tj, sh.tpb, cus);
+    if (chunk <= 0 && wide) chunk = pair_chunk(sh, planes, cus);
     if (chunk <= 0) {
         chunk = chunk0;
         while (chunk > 4 && ceil_div(rows, tj) * sh.n_windows * ceil_div(planes, chunk) < waves_wanted) chunk /= 2;
@@ -397,11 +483,8 @@ SweepShape compute_sweep_shape(const pic_sim *s) {
         if (tj == 1 && chunk == 4 && sh.v <= 2 && ceil_div(rows, 1) * sh.n_windows * ceil_div(planes, 4) < (int64_t)cus * 8)
             chunk = 2;
     }
-    sh.tj = tj;
-    sh.n_jt = (int)ceil_div(rows, tj);
     sh.chunk = (int)std::min<int64_t>(chunk, planes);
     sh.n_ic = (int)ceil_div(planes, sh.chunk);
-    sh.tpb = tpb;
     return sh;
 }
 
@@ -427,7 +510,7 @@ const void *sweep_fn_typed(int tj, bool dep, bool reflect) {
                 return nullptr;
             }
         case 4:
-            if constexpr (V == 4) {
+            if constexpr (V == 4 || (V == 2 && sizeof(S) == 8)) {
                 return nullptr;
             } else {
                 return sweep_fn_tj<S, A, 4, V>(dep, reflect);
@@ -457,11 +540,12 @@ const void *sweep_fn_short(int ch, bool reflect) {
 // which runs no step past the chunk (the general march runs its plane
 // buffers' count); the arithmetic is the same.
 const void *sweep_fn(int prec, int v, int tj, bool dep, bool reflect, int chunk = 0) {
-    if (tj == 1 && dep && chunk >= 1 && chunk <= 2 && v <= 2) {
+    if (tj == 1 && dep && chunk >= 1 && chunk <= 2 && (prec == PIC_FP64 ? v == 1 : v <= 2)) {
         if (prec == PIC_FP64) return sweep_fn_short<double, double, 1>(chunk, reflect);
         return v == 2 ? sweep_fn_short<float, float, 2>(chunk, reflect) : sweep_fn_short<float, float, 1>(chunk, reflect);
     }
-    if (prec == PIC_FP64) return sweep_fn_typed<double, double, 1>(tj, dep, reflect);
+    if (prec == PIC_FP64)
+        return v == 2 ? sweep_fn_typed<double, double, 2>(tj, dep, reflect) : sweep_fn_typed<double, double, 1>(tj, dep, reflect);
     if (v == 4) return sweep_fn_typed<float, float, 4>(tj, dep, reflect);
     return v == 2 ? sweep_fn_typed<float, float, 2>(tj, dep, reflect) : sweep_fn_typed<float, float, 1>(tj, dep, reflect);
 }
@@ -1031,6 +1115,7 @@ size_t partial_cap_bytes(const pic_params *p) {
 int batch_tiling(pic_batch *b) {
     int v = 2, wpb = 8;
     for (pic_sim *s : b->sims) v = std::min(v, values_per_lane(s, 0));
+    if (b->prec == PIC_FP64) v = 1;  // the batched FP64 sweep takes one value per lane
     for (pic_sim *s : b->sims)
         wpb = std::min(wpb, (int)ceil_div(s->p.nzb - 2, pic::window_step(v, s->p.nzb - 2)));
     const int cus = b->sims[0]->cus;
@@ -1858,8 +1943,9 @@ int pic_set_tiling(pic_sim *s, int rows_per_tile, int planes_per_chunk, int valu
     if (rows_per_tile < 0 || planes_per_chunk < 0) return fail(PIC_E_INVALID, "negative tiling");
     if (values_per_lane < 0 || values_per_lane > 4 || values_per_lane == 3)
         return fail(PIC_E_INVALID, "values_per_lane must be 0, 1, 2 or 4");
-    if (values_per_lane >= 2 && s->prec != PIC_FP32)
-        return fail(PIC_E_INVALID, "%d values per lane needs FP32 fields", values_per_lane);
+    if (values_per_lane == 4 && s->prec != PIC_FP32) return fail(PIC_E_INVALID, "4 values per lane needs FP32 fields");
+    if (values_per_lane == 2 && s->prec == PIC_FP64 && rows_per_tile > 2)
+        return fail(PIC_E_INVALID, "two FP64 values per lane take rows_per_tile 0, 1 or 2");
     if (values_per_lane == 4 && rows_per_tile > 3)
         return fail(PIC_E_INVALID, "four values per lane takes rows_per_tile 0, 1, 2 or 3");
     if (rows_per_tile && rows_per_tile != 1 && rows_per_tile != 2 && rows_per_tile != 3 && rows_per_tile != 4 &&
@@ -2270,7 +2356,9 @@ int pic_adjoint_create_levels(pic_sim *s, int64_t checkpoint_every, int64_t segm
     ad->chunk = adjoint_chunk(s);
     const int64_t xy = s->p.no_xy * s->p.no_xy, tiles = gather_blocks(s), nblk = adjoint_blocks(ad);
     const size_t pair = 2 * (size_t)s->n_vox * 8;
-    const size_t sizes[] = {ad->n_outer * pair, (size_t)(ad->K + 1) * pair, 2 * pair, (size_t)ad->K * xy * 8,
+    // the segment store is swept in place: kFieldSlack past its last array, as
+    // past every carrier array (a 16-byte lane may read one value past a row)
+    const size_t sizes[] = {ad->n_outer * pair, (size_t)(ad->K + 1) * pair + pic::kFieldSlack, 2 * pair, (size_t)ad->K * xy * 8,
                             (size_t)ad->K * xy * 8, (size_t)ad->K * tiles * 2 * 8,
                             (size_t)nblk * 2 * pic::kAdjNV * 8, (size_t)pic::kAdjNV * 8,
                             (size_t)ad->T * 2 * 8, 64, (size_t)xy * 8, ad->M > 1 ? (size_t)ad->M * pair : 0};

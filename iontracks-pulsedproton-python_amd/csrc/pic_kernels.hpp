@@ -222,6 +222,7 @@ struct SweepArgs {
 };
 
 typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
 typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
@@ -290,7 +291,7 @@ constexpr int kBufferWord3 = 0x00020000;
 // marches any chunk in groups of the plane-buffer count.
 template <typename S, typename A, int TJ, int V, bool REFLECT, int CH = 0>
 __device__ __forceinline__ void sweep_body(const SweepArgs &a, const int logical, const bool DEP) {
-    static_assert(V == 1 || ((V == 2 || V == 4) && sizeof(S) == 4), "two or four values per lane are FP32 layouts");
+    static_assert(V == 1 || V == 2 || (V == 4 && sizeof(S) == 4), "four values per lane are an FP32 layout");
     constexpr int WIN = window_outputs(V);  // k step between windows (V = 2: a.win_step, 124 or 126)
     // a block holds tpb consecutive j tiles of one chunk, each with wpb
     // windows: waves [q * wpb, (q + 1) * wpb) march tile q of the group, so
@@ -311,13 +312,15 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &a, const int logical
 
     double acc_r = 0.0, acc_p = 0.0, acc_n = 0.0;
     // The tiles of a block march in lockstep, one barrier per plane, when
-    // every wave of the block marches (a block-uniform condition: all its
-    // waves then run the same planes and reach every barrier). Left to
+    // every wave of the block marches: the block is full-sized, its window
+    // group holds wpb windows and its tile group tpb tiles (a block-uniform
+    // condition: all its waves then run the same planes and reach every
+    // barrier). Left to
     // themselves, neighbouring tiles drift apart along i and the halo rows
     // one loads are gone from the L2 by the time the other needs them:
     // measured on the 5 um full electrode, two tiles per block of 32 planes
     // take 1.58 ms per step in lockstep, 1.70 ms free-running.
-    const bool lockstep = tpb > 1 && (int)(blockDim.x >> 6) == a.wpb * tpb && a.n_windows % a.wpb == 0 &&
+    const bool lockstep = tpb > 1 && (int)(blockDim.x >> 6) == a.wpb * tpb && (kg + 1) * a.wpb <= a.n_windows &&
                           (jg + 1) * tpb <= a.n_jt;
     if (wave_id < a.wpb * tpb && win < a.n_windows && jt < a.n_jt) {
         const int no_xy = a.no_xy, nzb = a.nzb, pitch = a.pitch;
@@ -325,7 +328,9 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &a, const int logical
         const int k0 = win * step + V * lane;
         // k0 is a multiple of V; the pitch is a multiple of two (FP32), so a
         // lane with k0 < nzb reads at most the row's padding or, with four
-        // values on a row of 2 mod 4 values, two values past the row
+        // values on a row of 2 mod 4 values, two values past the row; with two
+        // FP64 values on a row of odd length (FP64 rows are not padded), one
+        // value past the row -- the next row's first, or slack past the array
         const bool k_in = k0 < nzb;
         bool k_out[V], k_gap[V], k_score[V];
 #pragma unroll
@@ -404,6 +409,14 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &a, const int logical
                     vp[v] = tp[v];
                     vn[v] = tn[v];
                 }
+            } else if constexpr (V == 2 && sizeof(S) == 8) {
+                // two FP64 values: 16 bytes per lane
+                const f64x2 tp = __builtin_bit_cast(f64x2, __builtin_amdgcn_raw_buffer_load_b128(rp, vo_load, so, 0));
+                const f64x2 tn = __builtin_bit_cast(f64x2, __builtin_amdgcn_raw_buffer_load_b128(rn, vo_load, so, 0));
+                vp[0] = tp.x;
+                vp[1] = tp.y;
+                vn[0] = tn.x;
+                vn[1] = tn.y;
             } else if constexpr (V == 2) {
                 const f32x2 tp = __builtin_bit_cast(f32x2, __builtin_amdgcn_raw_buffer_load_b64(rp, vo_load, so, 0));
                 const f32x2 tn = __builtin_bit_cast(f32x2, __builtin_amdgcn_raw_buffer_load_b64(rn, vo_load, so, 0));
@@ -497,10 +510,17 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &a, const int logical
                 // or nzb-1, a k end plane, which must stay zero; single values
                 // only at the edges of 126-value windows
                 if (pair_store) {
-                    const f32x2 wp = {k_out[0] ? vp[0] : 0.0f, k_out[1] ? vp[1] : 0.0f};
-                    const f32x2 wn = {k_out[0] ? vn[0] : 0.0f, k_out[1] ? vn[1] : 0.0f};
-                    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, wp), rp, vo_store, so, 0);
-                    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, wn), rn, vo_store, so, 0);
+                    if constexpr (sizeof(S) == 8) {
+                        const f64x2 wp = {k_out[0] ? vp[0] : 0.0, k_out[1] ? vp[1] : 0.0};
+                        const f64x2 wn = {k_out[0] ? vn[0] : 0.0, k_out[1] ? vn[1] : 0.0};
+                        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, wp), rp, vo_store, so, 0);
+                        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, wn), rn, vo_store, so, 0);
+                    } else {
+                        const f32x2 wp = {k_out[0] ? vp[0] : 0.0f, k_out[1] ? vp[1] : 0.0f};
+                        const f32x2 wn = {k_out[0] ? vn[0] : 0.0f, k_out[1] ? vn[1] : 0.0f};
+                        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, wp), rp, vo_store, so, 0);
+                        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, wn), rn, vo_store, so, 0);
+                    }
                     return;
                 }
             }

@@ -40,11 +40,16 @@ PARAMS = ("alpha", "mu_positive", "mu_negative", "D_positive", "D_negative", "tr
 
 
 class KsModel:
-    """k_s of one configuration as a function of theta (see the module)."""
+    """k_s of one configuration as a function of theta (see the module).
+
+    ``tiling`` is the forward sweep's tile shape, as in run_simulation_hip:
+    None for the library's rule or ``(rows_per_tile, planes_per_chunk,
+    values_per_lane)``; every tiling gives the same forward fields."""
 
     def __init__(self, config, rng: Optional[np.random.Generator] = None, checkpoint_every: int = 64,
-                 segments_per_outer: int = 0):
+                 segments_per_outer: int = 0, tiling: Optional[tuple] = None):
         self.config = config
+        self.tiling = tuple(int(v) for v in tiling) if tiling is not None else None
         self.schedule, self.xs, self.ys = draw_tracks(config, rng)
         self.theta0 = np.array([RECOMBINATION_ALPHA_CM3_S, config.mu_positive, config.mu_negative,
                                 config.D_positive, config.D_negative, config.track_radius_cm], dtype=np.float64)
@@ -76,6 +81,8 @@ class KsModel:
         """k_s at theta (one forward run, FP64 fields)."""
         T = self.config.total_time_steps
         with HipSimulation(self.config, "fp64", params=self.params(theta)) as sim:
+            if self.tiling:
+                sim.set_tiling(*self.tiling)
             sim.upload_tracks(self.schedule, self.xs, self.ys)
             sim.run_steps(0, T)
             per = sim.read_per_step(0, T)
@@ -90,6 +97,8 @@ class KsModel:
         d_rec = np.zeros(11)
         d_inj = np.zeros(2)
         with HipSimulation(self.config, "fp64", params=self.params(theta)) as sim:
+            if self.tiling:
+                sim.set_tiling(*self.tiling)
             sim.upload_tracks(self.schedule, self.xs, self.ys)
             adj = ctypes.c_void_p()
             _lib.check(L.pic_adjoint_create_levels(sim.handle, self.checkpoint_every, self.segments_per_outer,

@@ -47,11 +47,15 @@ def _load(path: Path) -> dict:
 
 def candidates(precision: str) -> list:
     """Tile shapes the library instantiates: rows 1, 2, 4 (FP32: 8 with one
-    value per lane, 1 to 3 with four), 4 to 32 planes per block."""
+    value per lane, 1 to 3 with four; FP64: 1 and 2 with two values per lane),
+    4 to 32 planes per block."""
     out = []
-    lanes = (4, 2, 1) if precision == "fp32" else (1,)
+    lanes = (4, 2, 1) if precision == "fp32" else (2, 1)
     for v in lanes:
-        rows = {1: (1, 2, 4, 8), 2: (1, 2, 4), 4: (1, 2, 3)}[v] if precision == "fp32" else (1, 2, 4)
+        if precision == "fp32":
+            rows = {1: (1, 2, 4, 8), 2: (1, 2, 4), 4: (1, 2, 3)}[v]
+        else:
+            rows = {1: (1, 2, 4), 2: (1, 2)}[v]
         for tj in rows:
             for chunk in (4, 8, 16, 32):
                 out.append((tj, chunk, v))

@@ -60,8 +60,9 @@ def test_short_march_matches_the_general_march(precision, values, wall):
 
 
 def test_default_tiling_follows_the_grid():
-    """Four values per lane (3-row tiles) only on grids with enough waves for
-    them; automatic values fall back to two under forced 4-row tiles; the
+    """Four FP32 values per lane (3-row tiles) and two FP64 values (2-row
+    tiles) only on grids with enough waves for them; automatic values fall
+    back to two FP32 values and one FP64 value under forced 4-row tiles; the
     default four-value run matches two values per lane bit for bit."""
     from configs import readme_config
     from pulsed_ion_chamber_hip.pulses import draw_tracks
@@ -71,8 +72,13 @@ def test_default_tiling_follows_the_grid():
     large = readme_config(5.0, 0.065)  # 268^2 x 406
     with HipSimulation(small, "fp32") as sim:
         assert sim.get_tiling()["values_per_lane"] == 2
-    with HipSimulation(large, "fp64") as sim:
+    with HipSimulation(small, "fp64") as sim:
         assert sim.get_tiling()["values_per_lane"] == 1
+    with HipSimulation(large, "fp64") as sim:
+        tiling = sim.get_tiling()
+        assert (tiling["rows_per_tile"], tiling["values_per_lane"]) == (2, 2)
+        sim.set_tiling(4, 16, 0)
+        assert sim.get_tiling() == dict(rows_per_tile=4, planes_per_chunk=16, values_per_lane=1)
     draw = draw_tracks(large)
     out = []
     with HipSimulation(large, "fp32") as sim:
