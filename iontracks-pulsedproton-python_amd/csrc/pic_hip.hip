@@ -318,25 +318,28 @@ int values_per_lane(const pic_sim *s, int force_v) {
 }
 
 // Planes per block of the four-value tiling, whose blocks hold two waves per
-// SIMD (8 per CU): the longest chunk (at most 32 planes) that still gives
-// four rounds of resident waves, so no CU marches much more than another;
-// failing that, the shortest chunk whose waves fit in one round (the most
-// waves that run at once). Measured on MI355X (3-row tiles, two per block,
-// each grid against its other chunks on one device): 1066^2 x 410, 32 planes
-// 1.570 ms (48: 1.591, 64: 1.593, 96: 1.653); 726^2 x 410, 32 (8 and 16
-// within 0.5 %); 486^2 x 410, 16 planes 332 us (8: 337, 32: 358); 326^2 x 410,
-// 8 planes 156 us (16: 166, 32: 188); 206^2 x 410 and 208^2 x 406, 16 planes
-// 67.8 and 69.7 us (8: 71.3 and 72.2, 32: 74.8 and 77.8); 536^2 x 210, 8
-// planes 227 us (16: 232, 32: 259); 366^2 x 210, 4 planes 116 us (8: 117,
-// 16: 128).
+// SIMD (8 per CU): the longest chunk of 40, 32, 16, 8 or 4 planes that still
+// gives four rounds of resident waves, so no CU marches much more than
+// another; failing that, the shortest chunk whose waves fit in one round (the
+// most waves that run at once), searched over every length from 4 to 32
+// planes: the busiest CU's blocks each march chunk + 2 planes, so the
+// shortest chunk that still fits one round is the fastest. Measured on MI355X
+// (3-row tiles, two per block, each grid against its other chunks on one
+// device): 1066^2 x 410, 40 planes 1.561 ms (32: 1.569, 36: 1.562, 44:
+// 1.586, 48: 1.578, 64: 1.579, 24: 1.579); 726^2 x 410, 40 planes 731 us
+// (32: 745, 36: 734, 44: 776, 48: 747); 486^2 x 410, 16 planes 332 us (8:
+// 337, 32: 358); 326^2 x 410, 8 planes 156 us (16: 166, 32: 188); 208^2 x
+// 406, 15 planes (490 blocks for 512 places) 68.3 us (16, 455 blocks: 69.8;
+// 14, 525 blocks, a second round: 88.9; 12: 82.7); 206^2 x 410, 14 planes
+// 66.3 us (15: 66.1, 16: 66.8, 12: 74.4); 536^2 x 210, 8 planes 227 us (16:
+// 232, 32: 259); 366^2 x 210, 4 planes 116 us (8: 117, 16: 128).
 int four_value_chunk(int64_t lateral, int64_t n_windows, int wpb, int tj, int tpb, int cus) {
     const int64_t round = (int64_t)cus * 8;
-    for (int c = 32; c >= 4; c /= 2)
+    for (int c : {40, 32, 16, 8, 4})
         if (sweep_waves(lateral, n_windows, wpb, tj, tpb, c) >= 4 * round) return c;
-    int chunk = 32;
-    for (int c = 32; c >= 4; c /= 2)
-        if (sweep_waves(lateral, n_windows, wpb, tj, tpb, c) <= round) chunk = c;
-    return chunk;
+    for (int c = 4; c < 32; ++c)
+        if (sweep_waves(lateral, n_windows, wpb, tj, tpb, c) <= round) return c;
+    return 32;
 }
 
 // Planes per block of two-FP64-value tiles: the chunk whose busiest CU

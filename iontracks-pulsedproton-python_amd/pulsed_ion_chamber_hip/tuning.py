@@ -5,8 +5,8 @@ The library picks a tiling by a rule fitted to measurements on MI355X
 (enough waves for the chip, then the tallest tile and longest chunk). A grid
 far from the measured ones may prefer another shape, so this module measures:
 it runs the first in-pulse steps of the configuration's own schedule under
-each candidate tiling (HIP events around every sweep launch), keeps the
-fastest, and caches it in a JSON file keyed by device, library build, grid,
+each candidate tiling and under the library's own choice (HIP events around
+every sweep launch), keeps the fastest, and caches it in a JSON file keyed by device, library build, grid,
 precision and wall (a rebuilt library may offer other tilings). Every tiling gives the same field values -- each voxel is updated by
 the same arithmetic -- so tuning changes speed only (and, within rounding,
 the order of the scored partial sums).
@@ -99,6 +99,12 @@ def autotune(config, precision: str = "fp32", steps: int = 48, repeats: int = 2,
         sim.set_tiling(0, 0, 0)
         sim.run_steps(0, min(steps, 8))  # load the kernels
         default_us = time_current()
+        # the library's own choice competes too: it may take a shape outside
+        # the candidate list (the short march's 1- and 2-plane chunks, the
+        # one-round chunk lengths between the powers of two, 40 planes)
+        t = sim.get_tiling()
+        default_tiling = (t["rows_per_tile"], t["planes_per_chunk"], t["values_per_lane"])
+        measured[default_tiling] = default_us
         for tiling in (tilings or candidates(precision)):
             try:
                 sim.set_tiling(*tiling)
@@ -107,6 +113,7 @@ def autotune(config, precision: str = "fp32", steps: int = 48, repeats: int = 2,
             measured[tiling] = time_current()
     best = min(measured, key=measured.get)
     result = {"tiling": list(best), "sweep_us": measured[best], "default_us": default_us,
+              "default_tiling": list(default_tiling),
               "steps": steps, "measured": {",".join(map(str, k)): v for k, v in measured.items()}}
     if use_cache:
         table = _load(path)
