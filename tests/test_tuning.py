@@ -125,3 +125,28 @@ def test_cache_key_names_the_library_build():
 
     cfg = SimulationConfig(**SMALL)
     assert f"lib {_lib.library_digest()}" in tuning.cache_key(cfg, "fp64")
+
+
+def test_one_round_grids_take_the_shortest_chunk_that_fits_one_round():
+    """The mid-size 208^2 x 406 grid: four-value 3-row pairs whose blocks all
+    fit one round of resident waves at 15 planes (490 blocks of 4 waves for
+    the 2048 waves the 256 CUs of an MI355X hold) but not at 14 (525): the
+    library takes 15, between the powers of two, and the fields equal those
+    of 16-plane chunks bit for bit."""
+    from configs import readme_config
+    from pulsed_ion_chamber_hip.pulses import draw_tracks
+    from pulsed_ion_chamber_hip.solver_hip import HipSimulation
+
+    cfg = readme_config(5.0, 0.05)
+    draw = draw_tracks(cfg)
+    out = []
+    with HipSimulation(cfg, "fp32") as sim:
+        assert sim.get_tiling() == dict(rows_per_tile=3, planes_per_chunk=15, values_per_lane=4)
+        sim.upload_tracks(*draw)
+        for t in ((0, 0, 0), (3, 16, 4)):
+            sim.set_tiling(*t)
+            sim.reset()
+            sim.run_steps(0, 40)
+            out.append(sim.read_fields())
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    np.testing.assert_array_equal(out[0][1], out[1][1])
