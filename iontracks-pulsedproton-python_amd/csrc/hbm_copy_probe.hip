@@ -5,7 +5,8 @@
 // With "all" it also times, over grids of 1024..65536 blocks, the same copy
 // with non-temporal loads and stores, a read-only pass over the two inputs
 // and a write-only pass over the two outputs: the ceilings of each traffic
-// mix on this device.
+// mix on this device; then the copy with four elements per thread in flight
+// in blocks of 1024 threads, plain and non-temporal.
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -37,6 +38,39 @@ __global__ void __launch_bounds__(256) copy2_nt(const f32x4 *__restrict__ a, con
         const f32x4 x = __builtin_nontemporal_load(&a[e]), y = __builtin_nontemporal_load(&b[e]);
         __builtin_nontemporal_store(x, &c[e]);
         __builtin_nontemporal_store(y, &d[e]);
+    }
+}
+
+// the copy with four 16-byte elements per thread and array in flight before
+// any store, for blocks of up to 1024 threads: does more in flight per thread
+// move the ceiling?
+template <bool NT>
+__global__ void __launch_bounds__(1024) copy2_x4(const f32x4 *__restrict__ a, const f32x4 *__restrict__ b,
+                                                 f32x4 *__restrict__ c, f32x4 *__restrict__ d, long n) {
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long e0 = blockIdx.x * (long)blockDim.x + threadIdx.x; e0 < n; e0 += 4 * stride) {
+        f32x4 x[4], y[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const long e = e0 + u * stride;
+            if (e < n) {
+                x[u] = NT ? __builtin_nontemporal_load(&a[e]) : a[e];
+                y[u] = NT ? __builtin_nontemporal_load(&b[e]) : b[e];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const long e = e0 + u * stride;
+            if (e < n) {
+                if (NT) {
+                    __builtin_nontemporal_store(x[u], &c[e]);
+                    __builtin_nontemporal_store(y[u], &d[e]);
+                } else {
+                    c[e] = x[u];
+                    d[e] = y[u];
+                }
+            }
+        }
     }
 }
 
@@ -108,6 +142,23 @@ int main(int argc, char **argv) {
             const double moved = (kind <= 1 ? 4.0 : 2.0) * bytes;
             printf("{\"kernel\": \"%s\", \"grid\": %d, \"avg_ms\": %.5f, \"GBs\": %.1f}\n", names[kind], g, ms,
                    moved / (ms * 1e-3) / 1e9);
+        }
+    // four elements in flight per thread, blocks of 1024 threads
+    for (int nt = 0; nt < 2; ++nt)
+        for (int g : {8192, 32768, 65536}) {
+            auto launch = [&]() {
+                if (nt) hipLaunchKernelGGL(copy2_x4<true>, dim3(g), dim3(1024), 0, 0, a, b, c, d, n);
+                else hipLaunchKernelGGL(copy2_x4<false>, dim3(g), dim3(1024), 0, 0, a, b, c, d, n);
+            };
+            for (int it = 0; it < 3; ++it) launch();
+            CHECK(hipEventRecord(e0));
+            for (int it = 0; it < reps; ++it) launch();
+            CHECK(hipEventRecord(e1));
+            CHECK(hipEventSynchronize(e1));
+            CHECK(hipEventElapsedTime(&ms, e0, e1));
+            ms /= reps;
+            printf("{\"kernel\": \"%s\", \"grid\": %d, \"block\": 1024, \"avg_ms\": %.5f, \"GBs\": %.1f}\n",
+                   nt ? "copy_x4_nontemporal" : "copy_x4", g, ms, 4.0 * bytes / (ms * 1e-3) / 1e9);
         }
     return 0;
 }
