@@ -113,6 +113,25 @@ def test_rejects_impossible_options():
         run_simulation_hip(fast_config(), progress=False, precision="fp16")
 
 
+def test_a_grid_beyond_device_memory_is_refused_before_allocating():
+    """The 1 um full electrode (5306^2 x 2010 = 56.6 G voxels, 905 GB of FP32
+    fields) cannot fit one MI355X: the handle is refused with MemoryError
+    from the free-memory check, before any device allocation, and the next
+    handle on the device works."""
+    import warnings
+
+    from configs import aic_config
+    from pulsed_ion_chamber_hip.solver_hip import HipSimulation
+
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")  # the 1 um dt resolves the beam's RF period
+        cfg = aic_config(0.265, 1.0, max_voxels=1e12)
+    with pytest.raises(MemoryError, match="device memory"):
+        HipSimulation(cfg, "fp32")
+    res = run_simulation_hip(fast_config(), progress=False, precision="fp32")
+    assert np.isfinite(res.ks)
+
+
 def test_fp32_fields_keep_ks_within_1e6():
     cfg = SimulationConfig(**SMALL, **TWO_SPECIES)
     ref = batched.run_batched(cfg)
