@@ -174,6 +174,14 @@ int pic_set_tiling(pic_sim *sim, int rows_per_tile, int planes_per_chunk, int va
  * block): neighbouring tiles then run on one CU side by side. */
 int pic_set_tile_group(pic_sim *sim, int tiles_per_block);
 
+/* Time steps per sweep launch: 1, or 2 for FP32 fields with the reflecting
+ * wall (pairs of steps in one pass over the fields, bit-identical to single
+ * steps; a run's odd step and a gather segment's odd step take one); 0 = the
+ * library's choice. planes_per_chunk (0 = keep) sets the two-step pass's
+ * planes per block. No counterpart in the reference, whose backends take one
+ * step per sweep (solver_cuda.py:834-863); a performance knob. */
+int pic_set_steps_per_pass(pic_sim *sim, int steps, int planes_per_chunk);
+
 /* The tiling the next sweep launch uses: the rule's choice for every knob
  * pic_set_tiling left at 0 (replaces reading back the threads per block the
  * reference's set_threads_per_block stores, solver_cuda.py:121-132). */

@@ -25,6 +25,7 @@
 
 #include "pic_hip.h"
 #include "pic_kernels.hpp"
+#include "pic_sweep2.hpp"
 #include "pic_adjoint.hpp"
 
 // Staging of pic_run_pulse_static, the entry point that allocates nothing:
@@ -127,6 +128,13 @@ struct pic_sim {
     int64_t n_tracks = 0;
     std::vector<int64_t> offsets;
     int force_tj = 0, force_chunk = 0, force_v = 0, force_tpb = 0;
+    int steps_per_pass = 0;        // 1, 2 (pairs of steps through sweep2_kernel), 0: the library's choice
+    int pass2_chunk = 0;           // planes per block of sweep2_kernel, 0: the library's choice
+    double *pair_part = nullptr;   // [2 launches][2 steps][pair_part_cap][3]: the two-step pass's partials
+    int64_t pair_part_cap = 0;
+    int pair_slot = 0;             // which half of pair_part the next pass writes
+    pic::FoldArgs pair_pending[2]{};  // the last pass's two steps, not yet folded into their records
+    bool pair_open = false;
     int cus = 256;                 // compute units of the device (queried once)
     bool shape_valid = false;      // `shape` matches the tiling knobs
     SweepShape shape;
@@ -872,11 +880,15 @@ int make_sweep_args(pic_sim *s, const SweepShape &sh, int64_t step, bool dep, co
     return PIC_OK;
 }
 
+int flush_pair(pic_sim *s, hipStream_t st);
+
 int enqueue_sweep(pic_sim *s, int64_t step, bool dep, const double *dens, const double *dep_part, int n_dep,
                   double *out4) {
     const SweepShape sh = sweep_shape(s);
     pic::SweepArgs a;
-    int rc = make_sweep_args(s, sh, step, dep, dens, dep_part, n_dep, out4, a);
+    int rc = flush_pair(s, nullptr);  // a pass's two steps, folded before this step takes the fold slot
+    if (rc) return rc;
+    rc = make_sweep_args(s, sh, step, dep, dens, dep_part, n_dep, out4, a);
     if (rc) return rc;
     // one extra block folds the previous step when one is pending
     const dim3 grid((unsigned)(sh.blocks() + (a.prev_out4 ? 1 : 0))), block((unsigned)sh.threads());
@@ -893,8 +905,154 @@ int enqueue_sweep(pic_sim *s, int64_t step, bool dep, const double *dens, const 
     return emit(s, KLaunch::make(f, grid, block, 0, a), s->stream);
 }
 
+// Two steps per pass (sweep2_kernel): blocks of 16 one-row waves, 12 output
+// rows, one k window of 240 values (248 in the first), `chunk` planes.
+struct Pass2Shape {
+    int n_windows = 0, n_jg = 0, n_ic = 0, chunk = 0;
+    int64_t blocks() const { return (int64_t)n_windows * n_jg * n_ic; }
+};
+
+// Planes per block of the two-step pass: one 16-wave block per CU, so the
+// chunk whose busiest CU marches the fewest planes, counted as rounds of
+// blocks x (chunk + 3) (a pass loads three planes before its first output).
+// Measured on MI355X (sweep time per step, each grid against single steps on
+// one device): 1066^2 x 410, 40 planes 1.41 ms against 1.56 (64: 1.41);
+// 726^2 x 410, 24 planes 689 us against 727 (40: 718, 64: 691); 506^2 x 1010,
+// 40 planes 803 against 965 (24: 811, 16: 838); 486^2 x 410, 16-24 planes
+// 323 against 331 (40: 357); 536^2 x 210, 16 planes 209 against 226 (24: 228);
+// 246^2 x 210, 24 planes 40.6 against 52.7 (8: 50.1, 16: 54.0); 206^2 x 410,
+// 16 planes 58.6 against 66.0 (8: 67.2, 24: 78.5).
+Pass2Shape pass2_shape(const pic_sim *s) {
+    Pass2Shape sh;
+    const int64_t lateral = s->p.no_xy - 2;
+    sh.n_windows = pic::sweep2_windows((int)s->p.nzb);
+    sh.n_jg = (int)ceil_div(lateral, pic::kS2Rows);
+    auto with_chunk = [&](int c) {
+        Pass2Shape t = sh;
+        t.chunk = (int)std::min<int64_t>(c, lateral);
+        t.n_ic = (int)ceil_div(lateral, t.chunk);
+        return t;
+    };
+    if (s->pass2_chunk > 0) return with_chunk(s->pass2_chunk);
+    Pass2Shape best = with_chunk(8);
+    double best_cost = -1;
+    for (int c : {8, 12, 16, 24, 32, 40, 48, 64}) {
+        const Pass2Shape t = with_chunk(c);
+        const double cost = (double)ceil_div(t.blocks(), s->cus) * (t.chunk + 3);
+        if (best_cost < 0 || cost < best_cost) {
+            best = t;
+            best_cost = cost;
+        }
+    }
+    return best;
+}
+
+// why a handle cannot take two steps per pass, or null
+const char *pass2_unsupported(const pic_sim *s) {
+    if (s->prec != PIC_FP32) return "two steps per pass need FP32 fields";
+    if (s->p.wall != PIC_WALL_REFLECTING) return "two steps per pass need the reflecting lateral wall";
+    if (s->p.no_xy < 4 || s->p.nzb < 4) return "two steps per pass need at least 2 interior voxels per axis";
+    return nullptr;
+}
+
+// Whether run_steps pairs steps: when asked to, or by default where the pass
+// is possible, no single-step tiling is forced, and its grid has at least 32
+// (row group, window) columns of blocks -- every grid measured from 206^2 x
+// 410 up ran faster per step.
+bool pass2_on(const pic_sim *s) {
+    if (s->steps_per_pass == 1 || pass2_unsupported(s)) return false;
+    if (s->steps_per_pass == 2) return true;
+    if (s->force_tj || s->force_chunk || s->force_v || s->force_tpb) return false;
+    const Pass2Shape sh = pass2_shape(s);
+    return (int64_t)sh.n_jg * sh.n_windows >= 32;
+}
+
+// The pass's partials buffer, grown to hold nblk blocks (between runs only).
+int ensure_pair_partials(pic_sim *s, int64_t nblk) {
+    if (nblk <= s->pair_part_cap) return PIC_OK;
+    if (s->pair_open) return fail(PIC_E_INVALID, "cannot grow the pass partials while a pass is pending");
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    double *grown = nullptr;
+    if (hipMalloc(&grown, (size_t)(4 * nblk * 3) * sizeof(double)) != hipSuccess)
+        return fail(PIC_E_NOMEM, "hipMalloc of %lld two-step partials failed", (long long)nblk);
+    if (s->pair_part) hipFree(s->pair_part);
+    s->pair_part = grown;
+    s->pair_part_cap = nblk;
+    return PIC_OK;
+}
+
+// fold the last pass's two steps into their records (one launch, a block each)
+int flush_pair(pic_sim *s, hipStream_t st) {
+    if (!s->pair_open) return PIC_OK;
+    pic::Fold2Args f{};
+    f.step[0] = s->pair_pending[0];
+    f.step[1] = s->pair_pending[1];
+    s->pair_open = false;
+    return emit(s, KLaunch::make((const void *)pic::fold2_kernel, dim3(2), dim3(256), 2, f), st ? st : s->stream);
+}
+
+// Steps `step` and `step + 1` in one sweep2_kernel launch. Its first blocks
+// fold what the previous launch left (a single step, or a pass's two steps);
+// its own two steps are folded by the next launch or the run's end.
+int enqueue_sweep2(pic_sim *s, int64_t step, const bool dep[2], const double *dens[2], const double *dep_part[2],
+                   int n_dep, double *out4[2]) {
+    const Pass2Shape sh = pass2_shape(s);
+    const int64_t nblk = sh.blocks();
+    if (nblk > s->pair_part_cap) return fail(PIC_E_INVALID, "two-step grid %lld exceeds its partial buffer", (long long)nblk);
+    pic::Sweep2Args a{};
+    a.n_prev = 0;
+    if (s->pending.out4) {
+        a.prev[a.n_prev++] = pic::FoldArgs{s->pending.part, s->pending.dep, s->pending.out4, s->pending.n, s->pending.n_dep};
+        s->pending = pic_sim::Pending{};
+    }
+    if (s->pair_open) {
+        a.prev[a.n_prev++] = s->pair_pending[0];
+        a.prev[a.n_prev++] = s->pair_pending[1];
+        s->pair_open = false;
+    }
+    double *base = s->pair_part + (int64_t)s->pair_slot * 2 * s->pair_part_cap * 3;
+    s->pair_slot ^= 1;
+    a.pos = static_cast<const float *>(s->cur_p);
+    a.neg = static_cast<const float *>(s->cur_n);
+    a.pos_out = static_cast<float *>(s->nxt_p);
+    a.neg_out = static_cast<float *>(s->nxt_n);
+    a.dens1 = dep[0] ? dens[0] : s->zero_dens;
+    a.dens2 = dep[1] ? dens[1] : s->zero_dens;
+    a.part1 = base;
+    a.part2 = base + s->pair_part_cap * 3;
+    a.no_xy = (int)s->p.no_xy;
+    a.nzb = (int)s->p.nzb;
+    a.nze = (int)s->p.nze;
+    a.no_z = (int)s->p.no_z;
+    a.pitch = (int)s->pitch;
+    a.n_windows = sh.n_windows;
+    a.n_jg = sh.n_jg;
+    a.n_ic = sh.n_ic;
+    a.chunk = sh.chunk;
+    a.mid = s->p.mid_xy;
+    a.rsq = s->p.scoring_radius_sq;
+    a.p_lat = s->p.p_lat;
+    a.p_zm = s->p.p_zm;
+    a.p_zp = s->p.p_zp;
+    a.n_lat = s->p.n_lat;
+    a.n_zm = s->p.n_zm;
+    a.n_zp = s->p.n_zp;
+    a.alpha_dt = s->p.alpha_dt;
+    const int rc = emit(s,
+                        KLaunch::make((const void *)pic::sweep2_kernel, dim3((unsigned)(nblk + a.n_prev)),
+                                      dim3(64 * pic::kS2Waves), 0, a),
+                        s->stream);
+    if (rc) return rc;
+    for (int q = 0; q < 2; ++q)
+        s->pair_pending[q] = pic::FoldArgs{q == 0 ? a.part1 : a.part2, dep[q] ? dep_part[q] : nullptr, out4[q],
+                                           (int)nblk, dep[q] ? n_dep : 0};
+    s->pair_open = true;
+    return PIC_OK;
+}
+
 // fold the pending step's partials into its record (end of a run of steps)
 int flush_pending(pic_sim *s, hipStream_t st = nullptr) {
+    if (s->pair_open) return flush_pair(s, st);  // a pass and a single step are never pending together
     if (!s->pending.out4) return PIC_OK;
     close_group(s);
     record(s, 2);
@@ -903,6 +1061,7 @@ int flush_pending(pic_sim *s, hipStream_t st = nullptr) {
     HIP_LAUNCH_CHECK();
     record(s, 2);
     s->pending = pic_sim::Pending{};
+    s->pair_open = false;
     return PIC_OK;
 }
 
@@ -912,10 +1071,10 @@ int64_t gather_blocks(const pic_sim *s) {
 }
 
 // Density and deposition-partial slots: step t uses slot t % dens_slots. Two
-// gather batches and one step: the gather of the next segment runs while this
+// gather batches and two steps: the gather of the next segment runs while this
 // segment's sweeps read theirs, and this segment's first launch still folds
-// the previous segment's last step.
-int64_t dens_slots(int64_t batch) { return 2 * batch + 1; }
+// the previous segment's last step, or its last two after a two-step pass.
+int64_t dens_slots(int64_t batch) { return 2 * batch + 2; }
 
 // arguments of the gather of steps from `first` (slot = step mod dens_slots);
 // offsets are device offsets into xs/ys
@@ -1092,10 +1251,13 @@ int check_params(const pic_params *p, int precision) {
     return PIC_OK;
 }
 
-// steps per gather launch: as many as fit in ~256 MB of density, at most 128
+// steps per gather launch: as many as fit in ~256 MB of density, at most
+// 128, and an even count where more than one fits, so that two steps per
+// pass pair every step of a segment
 int64_t batch_steps(const pic_params *p) {
     const int64_t per = p->no_xy * p->no_xy * 8;
-    return std::max<int64_t>(1, std::min<int64_t>(128, (int64_t)(256LL << 20) / per));
+    const int64_t n = std::max<int64_t>(1, std::min<int64_t>(128, (int64_t)(256LL << 20) / per));
+    return n > 1 ? n & ~(int64_t)1 : n;
 }
 
 size_t partial_cap_bytes(const pic_params *p) {
@@ -1344,10 +1506,12 @@ int adjoint_recompute_sweep(pic_adjoint *ad, int64_t t, int64_t src) {
     const int n_tiles = (int)gather_blocks(s);
     const int64_t slot = t % dens_slots(s->batch);
     s->pending = pic_sim::Pending{};
+    s->pair_open = false;
     pic::SweepArgs a;
     int rc = make_sweep_args(s, sh, 0, dep, s->dens + slot * xy, s->dep_part + slot * n_tiles, n_tiles, ad->scratch, a);
     if (rc) return rc;
     s->pending = pic_sim::Pending{};
+    s->pair_open = false;
     double *in = ad->store + src * 2 * s->n_vox;
     a.pos = in;
     a.neg = in + s->n_vox;
@@ -1522,6 +1686,12 @@ int pic_create(const pic_params *p, int precision, pic_sim **out) {
         hipMalloc(&s->dep_part, slots * gather_blocks(s) * 8) != hipSuccess ||
         hipMalloc(&s->scratch4, 8 * 8) != hipSuccess || hipMalloc(&s->zero_dens, xy * 8) != hipSuccess)
         return cleanup(fail(PIC_E_NOMEM, "hipMalloc of scratch failed"));
+    // the two-step pass's partials, for its default chunk (a forced chunk
+    // grows them in pic_set_steps_per_pass)
+    if (!pass2_unsupported(s)) {
+        const int r2 = ensure_pair_partials(s, pass2_shape(s).blocks());
+        if (r2) return cleanup(r2);
+    }
     if (hipMemsetAsync(s->dens, 0, slots * xy * 8, s->stream) != hipSuccess ||
         hipMemsetAsync(s->zero_dens, 0, xy * 8, s->stream) != hipSuccess ||
         hipMemsetAsync(s->scratch4, 0, 64, s->stream) != hipSuccess)
@@ -1536,7 +1706,7 @@ int pic_destroy(pic_sim *s) {
     if (s->stream) hipStreamSynchronize(s->stream);
     if (s->gstream) hipStreamSynchronize(s->gstream);
     void *bufs[] = {s->cur_p, s->cur_n, s->nxt_p, s->nxt_n, s->dens, s->sweep_part, s->dep_part,
-                    s->scratch4, s->zero_dens, s->per_step, s->xs, s->ys, s->offsets_d};
+                    s->scratch4, s->zero_dens, s->per_step, s->xs, s->ys, s->offsets_d, s->pair_part};
     for (void *b : bufs)
         if (b) hipFree(b);
     free_bins(s);
@@ -1560,6 +1730,7 @@ int pic_upload_tracks(pic_sim *s, const double *xs, const double *ys, int64_t n_
     HIP_TRY(hipStreamSynchronize(s->stream));
     HIP_TRY(hipStreamSynchronize(s->gstream));  // a side gather reads the track arrays
     s->pending = pic_sim::Pending{};
+    s->pair_open = false;
     free_bins(s);
     if (s->xs) hipFree(s->xs);
     if (s->ys) hipFree(s->ys);
@@ -1681,6 +1852,7 @@ int run_steps_on(pic_sim *s, const TrackView &tv, int64_t first, int64_t count, 
         if (!r && hipEventRecord(s->seg_dens, s->gstream) != hipSuccess) r = fail(PIC_E_HIP, "hipEventRecord failed");
         return r ? r : 1;
     };
+    const bool use_pairs = pass2_on(s);
     bool gather_queued = false;
     if (!chained) {
         const int g = side_gather(first);
@@ -1698,11 +1870,26 @@ int run_steps_on(pic_sim *s, const TrackView &tv, int64_t first, int64_t count, 
             if (hipStreamWaitEvent(s->stream, s->seg_dens, 0) != hipSuccess) rc = fail(PIC_E_HIP, "hipStreamWaitEvent failed");
             gather_queued = false;
         }
-        for (int64_t step = b0; step < b1 && rc == PIC_OK; ++step) {
-            const bool dep = off[step + 1] > off[step];
-            const int64_t slot = step % slots;
-            rc = enqueue_sweep(s, step, dep, s->dens + slot * xy, s->dep_part + slot * n_tiles, n_tiles,
-                               tv.per_step + 4 * step);
+        for (int64_t step = b0, stride = 1; step < b1 && rc == PIC_OK; step += stride) {
+            stride = use_pairs && step + 1 < b1 ? 2 : 1;
+            if (stride == 2) {
+                bool dep[2];
+                const double *dens[2], *dpart[2];
+                double *out4[2];
+                for (int q = 0; q < 2; ++q) {
+                    const int64_t t = step + q, slot = t % slots;
+                    dep[q] = off[t + 1] > off[t];
+                    dens[q] = s->dens + slot * xy;
+                    dpart[q] = s->dep_part + slot * n_tiles;
+                    out4[q] = tv.per_step + 4 * t;
+                }
+                rc = enqueue_sweep2(s, step, dep, dens, dpart, n_tiles, out4);
+            } else {
+                const bool dep = off[step + 1] > off[step];
+                const int64_t slot = step % slots;
+                rc = enqueue_sweep(s, step, dep, s->dens + slot * xy, s->dep_part + slot * n_tiles, n_tiles,
+                                   tv.per_step + 4 * step);
+            }
             if (!rc) swap_buffers(s);
             if (!rc && !chained && step == b0 && b1 < end) {
                 if (hipEventRecord(s->seg_go, s->stream) != hipSuccess ||
@@ -1728,7 +1915,10 @@ int run_steps_on(pic_sim *s, const TrackView &tv, int64_t first, int64_t count, 
     if (hipEventRecord(s->seg_dens, s->gstream) != hipSuccess || hipStreamWaitEvent(s->stream, s->seg_dens, 0) != hipSuccess) {
         if (!rc) rc = fail(PIC_E_HIP, "ordering the handle's stream after its side stream failed");
     }
-    if (rc) s->pending = pic_sim::Pending{};  // a failed run leaves no step to fold
+    if (rc) {  // a failed run leaves no step to fold
+        s->pending = pic_sim::Pending{};
+        s->pair_open = false;
+    }
     return rc;
 }
 
@@ -1808,6 +1998,7 @@ int pic_run_pulse_static(pic_sim *s, const double *xs, const double *ys, const i
     // only after the copies above (same stream; the side stream's first
     // gather is queued inside run_steps_on after it synchronises both).
     s->pending = pic_sim::Pending{};
+    s->pair_open = false;
     rc = run_steps_on(s, tv, 0, n_steps, true);
     if (rc) return rc;
     HIP_TRY(hipMemcpyAsync(per_step_out, tv.per_step, (size_t)n_steps * 32, hipMemcpyDeviceToHost, s->stream));
@@ -1974,6 +2165,25 @@ int pic_set_tiling(pic_sim *s, int rows_per_tile, int planes_per_chunk, int valu
     return rc;
 }
 
+int pic_set_steps_per_pass(pic_sim *s, int steps, int planes_per_chunk) {
+    if (!s) return fail(PIC_E_INVALID, "sim is NULL");
+    if (steps < 0 || steps > 2) return fail(PIC_E_INVALID, "steps per pass must be 0, 1 or 2");
+    if (planes_per_chunk < 0) return fail(PIC_E_INVALID, "negative planes_per_chunk");
+    if (steps == 2) {
+        if (const char *why = pass2_unsupported(s)) return fail(PIC_E_INVALID, "%s", why);
+    }
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    const int old_steps = s->steps_per_pass, old_chunk = s->pass2_chunk;
+    s->steps_per_pass = steps;
+    if (planes_per_chunk > 0) s->pass2_chunk = planes_per_chunk;
+    const int rc = pass2_unsupported(s) ? PIC_OK : ensure_pair_partials(s, pass2_shape(s).blocks());
+    if (rc) {
+        s->steps_per_pass = old_steps;
+        s->pass2_chunk = old_chunk;
+    }
+    return rc;
+}
+
 int pic_set_tile_group(pic_sim *s, int tiles_per_block) {
     if (!s) return fail(PIC_E_INVALID, "sim is NULL");
     if (tiles_per_block < 0 || tiles_per_block > 8) return fail(PIC_E_INVALID, "tiles_per_block must be 0..8");
@@ -2030,6 +2240,7 @@ int pic_get_timing(pic_sim *s, double *sweep_ms, int64_t *sweep_launches, double
 int pic_reset(pic_sim *s) {
     if (!s) return fail(PIC_E_INVALID, "sim is NULL");
     s->pending = pic_sim::Pending{};
+    s->pair_open = false;
     void *bufs[4] = {s->cur_p, s->cur_n, s->nxt_p, s->nxt_n};
     const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(s->n_vox, 256), 16384));
     for (void *b : bufs) {
@@ -2170,7 +2381,10 @@ int pic_batch_run(pic_batch *b, int64_t first, int64_t count) {
     b->slot_busy[0] = b->slot_busy[1] = false;
     int rc = ensure_batch_slots(b, gbatch);
     if (rc) return rc;
-    for (pic_sim *s : b->sims) s->pending = pic_sim::Pending{};  // earlier runs were flushed
+    for (pic_sim *s : b->sims) {  // earlier runs were flushed
+        s->pending = pic_sim::Pending{};
+        s->pair_open = false;
+    }
     b->ev_used = 0;
     const dim3 block((unsigned)(64 * b->wpb));
     std::vector<int> active, cand_of;

@@ -231,6 +231,47 @@ typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 // included): 32-bit data format, no swizzle, no index stride.
 constexpr int kBufferWord3 = 0x00020000;
 
+// The FP32 voxel update (flux form): x + sum_w w (x_w - x), the two drift
+// terms as fused multiply-adds over the lateral sum. The weights sum to one,
+// so this is the same update as the reference's weighted sum, and written as
+// fluxes it moves charge between neighbours without creating any, whatever
+// the rounding of the FP32 weights. The recombination product and the final
+// x + flux - rec are FP64 (x + flux is exact there), so the stored value is
+// rounded to FP32 once: on the strongest-recombination point of a saturation
+// curve (50 V, 40 Gy/s) that keeps |dk_s| at 2.3e-7 where two FP32 roundings
+// reach 9.5e-7. n's drift runs the other way (km and kp swapped), as in the
+// reference (solver_numba_parallel.py:374-405).
+__device__ __forceinline__ void fp32_update(float p, float n, float p_km, float p_kp, float n_km, float n_kp,
+                                            float p_jm, float p_jp, float p_im, float p_ip, float n_jm, float n_jp,
+                                            float n_im, float n_ip, float f_pzm, float f_pzp, float f_plat,
+                                            float f_nzm, float f_nzp, float f_nlat, double alpha_dt, float &po,
+                                            float &no, double &rec) {
+    const float lp = ((p_jm - p) + (p_jp - p)) + ((p_im - p) + (p_ip - p));
+    const float ln = ((n_jm - n) + (n_jp - n)) + ((n_im - n) + (n_ip - n));
+    const float tp = __builtin_fmaf(f_pzm, p_km - p, __builtin_fmaf(f_pzp, p_kp - p, f_plat * lp));
+    const float tn = __builtin_fmaf(f_nzm, n_kp - n, __builtin_fmaf(f_nzp, n_km - n, f_nlat * ln));
+    rec = alpha_dt * (double)p * (double)n;
+    po = (float)(((double)p + (double)tp) - rec);
+    no = (float)(((double)n + (double)tn) - rec);
+}
+
+// Scored sums of a lane's four FP32 values, in FP64: the scored ones of the
+// four summed first, then added to the lane's sums.
+__device__ __forceinline__ void score4(const bool (&k_score)[4], const double (&rec)[4], const float (&po)[4],
+                                       const float (&no)[4], double &sr, double &sp, double &sn) {
+    double r = k_score[0] ? rec[0] : 0.0;
+    double pp = k_score[0] ? (double)po[0] : 0.0, nn = k_score[0] ? (double)no[0] : 0.0;
+#pragma unroll
+    for (int v = 1; v < 4; ++v) {
+        r += k_score[v] ? rec[v] : 0.0;
+        pp += k_score[v] ? (double)po[v] : 0.0;
+        nn += k_score[v] ? (double)no[v] : 0.0;
+    }
+    sr += r;
+    sp += pp;
+    sn += nn;
+}
+
 // One Lax-Wendroff step of both species, fused with the gap-layer deposition
 // of this step (DEP), the lateral-ring update of the next arrays and the
 // scored reduction.
@@ -654,39 +695,16 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &a, const int logical
                             po[v] = (S)((double)p_new - rec[v]);
                             no[v] = (S)((double)n_new - rec[v]);
                         } else {
-                            // FP32 fields: flux form x + sum_w w (x_w - x). The weights
-                            // sum to one, so this is the same update, and written as
-                            // fluxes it moves charge between neighbours without
-                            // creating any, whatever the rounding of the FP32 weights.
-                            // The recombination product and the final x + flux - rec
-                            // are FP64 (x + flux is exact there), so the stored value
-                            // is rounded to FP32 once: on the strongest-recombination
-                            // point of a saturation curve (50 V, 40 Gy/s) that keeps
-                            // |dk_s| at 2.3e-7 where two FP32 roundings reach 9.5e-7.
+                            // FP32 fields: the flux form (fp32_update)
                             const float p = cp[r + 1][v], n = cn[r + 1][v];
-                            const float tp = f_pzm * (p_km[v] - p) + f_pzp * (p_kp[v] - p) +
-                                             f_plat * (((cp[r][v] - p) + (cp[r + 2][v] - p)) +
-                                                       ((im_p - p) + (ip_p - p)));
-                            const float tn = f_nzm * (n_kp[v] - n) + f_nzp * (n_km[v] - n) +
-                                             f_nlat * (((cn[r][v] - n) + (cn[r + 2][v] - n)) +
-                                                       ((im_n - n) + (ip_n - n)));
-                            rec[v] = a.alpha_dt * (double)p * (double)n;
-                            po[v] = (float)(((double)p + (double)tp) - rec[v]);
-                            no[v] = (float)(((double)n + (double)tn) - rec[v]);
+                            fp32_update(p, n, p_km[v], p_kp[v], n_km[v], n_kp[v], cp[r][v], cp[r + 2][v], im_p,
+                                        ip_p, cn[r][v], cn[r + 2][v], im_n, ip_n, f_pzm, f_pzp, f_plat, f_nzm, f_nzp,
+                                        f_nlat, a.alpha_dt, po[v], no[v], rec[v]);
                         }
                     }
                     if (inside) {
                         if constexpr (kMerged) {
-                            double r = 0.0, pp = 0.0, nn = 0.0;
-#pragma unroll
-                            for (int v = 0; v < V; ++v) {
-                                r += k_score[v] ? rec[v] : 0.0;
-                                pp += k_score[v] ? (double)po[v] : 0.0;
-                                nn += k_score[v] ? (double)no[v] : 0.0;
-                            }
-                            sr[0] += r;
-                            sp[0] += pp;
-                            sn[0] += nn;
+                            score4(k_score, rec, po, no, sr[0], sp[0], sn[0]);
                         } else {
 #pragma unroll
                             for (int v = 0; v < V; ++v) {

@@ -16,6 +16,7 @@ PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent.parent
 SOURCES = [PKG.parent / "csrc" / "pic_hip.hip"]
 DEPENDS = SOURCES + [PKG.parent / "csrc" / "pic_kernels.hpp", PKG.parent / "csrc" / "pic_adjoint.hpp",
+                     PKG.parent / "csrc" / "pic_sweep2.hpp",
                      ROOT / "include" / "pic_hip.h"]
 OUTPUT = PKG / "libpic_hip.so"
 ARCH = os.environ.get("PIC_HIP_ARCH", "gfx950")

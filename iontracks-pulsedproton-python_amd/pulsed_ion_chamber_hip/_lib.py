@@ -52,6 +52,7 @@ EXPORTED = (
     "pic_get_gather_mode",
     "pic_set_tiling",
     "pic_set_tile_group",
+    "pic_set_steps_per_pass",
     "pic_get_tiling",
     "pic_enable_timing",
     "pic_get_timing",
@@ -147,6 +148,7 @@ def open_library():
         "pic_get_gather_mode": [_VP, ctypes.POINTER(ctypes.c_int)],
         "pic_set_tiling": [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int],
         "pic_set_tile_group": [_VP, ctypes.c_int],
+        "pic_set_steps_per_pass": [_VP, ctypes.c_int, ctypes.c_int],
         "pic_get_tiling": [_VP] + [ctypes.POINTER(ctypes.c_int)] * 3,
         "pic_enable_timing": [_VP, ctypes.c_int],
         "pic_get_timing": [_VP, _DP, _IP, _DP, _IP, _DP],
