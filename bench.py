@@ -30,8 +30,9 @@ in FP64 and FP32 beside that time and its published k_s or f.
 
 Prints one JSON line. ``value`` is interior voxel-steps per second over the
 timed region; ``roofline`` divides the sweep kernel's compulsory HBM bytes
-(read n+ and n-, write n+' and n-': 16 B per interior voxel in FP32, 32 B in
-FP64) by its average duration, measured with HIP events on the handle's
+(read n+ and n-, write n+' and n-': 16 B per interior voxel and step in FP32,
+32 B in FP64; a launch of the two-step pass, which FP32 grids with the
+reflecting wall take, counts both steps) by its average duration, measured with HIP events on the handle's
 stream over the timed region: one event pair around each run of 16
 back-to-back sweep launches, so every launch is timed (with the ~1 us
 between launches) while the pairs cost the run under 1 %; ``traffic`` is
