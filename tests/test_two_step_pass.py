@@ -113,3 +113,22 @@ def test_pairs_are_refused_where_the_pass_does_not_exist():
             sim.set_steps_per_pass(2)
         with pytest.raises(ValueError):
             sim.set_steps_per_pass(3)
+
+
+def test_pairs_replayed_through_graph_chains_equal_direct_launches():
+    """With graph replay, a segment's pass launches (two steps each, their
+    folds in the next launch's first blocks) are collected and replayed
+    through the chains: the same fields and records."""
+    cfg = aic_config(0.05, 5.0)
+    draw = draw_tracks(cfg)
+    out = []
+    for graphs in (False, True):
+        with HipSimulation(cfg, "fp32") as sim:
+            sim.set_graphs(graphs)
+            sim.upload_tracks(*draw)
+            sim.run_steps(0, 45)
+            sim.run_steps(45, 30)
+            out.append((sim.read_per_step(0, 75), sim.read_fields()))
+    np.testing.assert_array_equal(out[0][1][0], out[1][1][0])
+    np.testing.assert_array_equal(out[0][1][1], out[1][1][1])
+    np.testing.assert_array_equal(out[0][0], out[1][0])
