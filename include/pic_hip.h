@@ -174,7 +174,7 @@ int pic_set_tiling(pic_sim *sim, int rows_per_tile, int planes_per_chunk, int va
  * block): neighbouring tiles then run on one CU side by side. */
 int pic_set_tile_group(pic_sim *sim, int tiles_per_block);
 
-/* Time steps per sweep launch: 1, or 2 for FP32 fields with the reflecting
+/* Time steps per sweep launch: 1, or 2 for FP32 fields, either lateral
  * wall (pairs of steps in one pass over the fields, bit-identical to single
  * steps; a run's odd step and a gather segment's odd step take one); 0 = the
  * library's choice. planes_per_chunk (0 = keep) sets the two-step pass's

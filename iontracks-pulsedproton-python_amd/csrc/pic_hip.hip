@@ -950,7 +950,6 @@ Pass2Shape pass2_shape(const pic_sim *s) {
 // why a handle cannot take two steps per pass, or null
 const char *pass2_unsupported(const pic_sim *s) {
     if (s->prec != PIC_FP32) return "two steps per pass need FP32 fields";
-    if (s->p.wall != PIC_WALL_REFLECTING) return "two steps per pass need the reflecting lateral wall";
     if (s->p.no_xy < 4 || s->p.nzb < 4) return "two steps per pass need at least 2 interior voxels per axis";
     return nullptr;
 }
@@ -1038,6 +1037,7 @@ int enqueue_sweep2(pic_sim *s, int64_t step, const bool dep[2], const double *de
     a.n_zm = s->p.n_zm;
     a.n_zp = s->p.n_zp;
     a.alpha_dt = s->p.alpha_dt;
+    a.reflect = s->p.wall == PIC_WALL_REFLECTING ? 1 : 0;
     const int rc = emit(s,
                         KLaunch::make((const void *)pic::sweep2_kernel, dim3((unsigned)(nblk + a.n_prev)),
                                       dim3(64 * pic::kS2Waves), 0, a),

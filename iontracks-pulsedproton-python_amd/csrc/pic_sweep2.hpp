@@ -1,4 +1,4 @@
-// Two time steps per pass over FP32 fields with a reflecting lateral wall:
+// Two time steps per pass over FP32 fields, either lateral wall:
 // the field traffic of one step for two. Included once, by pic_hip.hip, after
 // pic_kernels.hpp.
 //
@@ -17,11 +17,15 @@
 //
 // Every voxel update is the single-step sweep's FP32 arithmetic
 // (fp32_update), so the fields equal two single steps bit for bit. The
-// lateral ring follows the single-step kernel's reflecting-wall rule: a ring
+// lateral ring follows the single-step kernel's rules. Reflecting: a ring
 // voxel reads its nearest interior column plus the deposition at the ring
 // position -- for T1 too: the ring row's T1 entry in LDS is written by the
 // interior row next to it, the ring plane's from the raw T1 of the plane
-// next to it.
+// next to it. Absorbing: a ring voxel is carried, its value plus each step's
+// deposition (T1 = step-t value + both depositions, which is also step t+2);
+// the ring row's own wave writes its T1 entry, the block holding row 1 or
+// no_xy-2 stores the ring row, the chunk holding plane 1 or no_xy-2 the ring
+// plane.
 #pragma once
 
 namespace pic {
@@ -75,6 +79,7 @@ struct Sweep2Args {
     // block each ahead of the sweep blocks (0, 1 or 2)
     FoldArgs prev[2];
     int n_prev;
+    int reflect;  // lateral wall: 1 reflecting, 0 absorbing
 };
 
 // number of k windows of the two-step pass over nzb values per row
@@ -103,6 +108,12 @@ __global__ void __launch_bounds__(1024) sweep2_kernel(Sweep2Args a) {
     const bool interior_row = r >= 1 && r <= no_xy - 2;
     const bool t1_row = w >= 1 && w <= kS2Waves - 2 && interior_row;
     const bool out_row = w >= 2 && w <= kS2Waves - 3 && interior_row;
+    const bool reflect = a.reflect != 0;
+    // absorbing wall: a ring row's wave carries its values through both steps
+    // (t-level value + both depositions); the block whose output rows reach
+    // the ring stores them
+    const bool ring_row = (r == 0 || r == no_xy - 1) && w >= 1 && w <= kS2Waves - 2;
+    const bool ring_store = !reflect && ring_row && ((r == 0 && jg == 0) || (r == no_xy - 1 && j0 + kS2Rows - 1 >= no_xy - 2));
     const int i0 = 1 + ic * a.chunk;
     const int i1 = min(i0 + a.chunk, no_xy - 1);
     const int k0 = kS2Step * kw + 4 * lane;
@@ -136,7 +147,8 @@ __global__ void __launch_bounds__(1024) sweep2_kernel(Sweep2Args a) {
     // step-t values of plane i, this wave's row: the nearest interior column
     // (reflecting wall), the deposition of step t+1 at (i, rl) added later
     auto load_t = [&](int i, f32x4 &vp, f32x4 &vn) {
-        const int si = min(max(i, 1), no_xy - 2), sj = min(max(rl, 1), no_xy - 2);
+        const int si = reflect ? min(max(i, 1), no_xy - 2) : min(max(i, 0), no_xy - 1);
+        const int sj = reflect ? min(max(rl, 1), no_xy - 2) : rl;
         vp = __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(rsrc(a.pos, si), vo_load, row_off(sj), 0));
         vn = __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(rsrc(a.neg, si), vo_load, row_off(sj), 0));
     };
@@ -219,7 +231,7 @@ __global__ void __launch_bounds__(1024) sweep2_kernel(Sweep2Args a) {
             add_gap(up, un, dens_at(a.dens2, q, r));
             ux[b][w][0][lane] = up;
             ux[b][w][1][lane] = un;
-            if (r == 1 || r == no_xy - 2) {
+            if (reflect && (r == 1 || r == no_xy - 2)) {
                 // the ring row next to this one: its T1 is this row's, plus the
                 // deposition at the ring position
                 const int ring = r == 1 ? 0 : no_xy - 1;
@@ -229,14 +241,43 @@ __global__ void __launch_bounds__(1024) sweep2_kernel(Sweep2Args a) {
                 ux[b][wr][0][lane] = mp;
                 ux[b][wr][1][lane] = mn;
             }
+        } else if (!reflect && ring_row) {
+            // absorbing: the ring row is carried, not swept
+            up = bp;
+            un = bn;
+            add_gap(up, un, dens_at(a.dens2, q, r));
+            ux[b][w][0][lane] = up;
+            ux[b][w][1][lane] = un;
         }
     };
-    // T1 of a ring plane (0 or no_xy-1): the raw T1 of the plane next to it,
-    // plus the deposition at the ring plane
-    auto t1_ring_plane = [&](int q, f32x4 &up, f32x4 &un) {
-        up = raw_p;
-        un = raw_n;
+    // T1 of a ring plane (0 or no_xy-1), plus the deposition at the ring
+    // plane: reflecting, the raw T1 of the plane next to it; absorbing, the
+    // plane's own step-t value (deposition of t+1 added), carried
+    auto t1_ring_plane = [&](int q, const f32x4 &tq_p, const f32x4 &tq_n, f32x4 &up, f32x4 &un) {
+        up = reflect ? raw_p : tq_p;
+        un = reflect ? raw_n : tq_n;
         add_gap(up, un, dens_at(a.dens2, q, rl));
+    };
+    // store the values of this wave's row at plane p (own lanes of the window)
+    auto store_row = [&](int p, const f32x4 &op, const f32x4 &on) {
+        if (k_out[0] || k_out[1] || k_out[2] || k_out[3]) {
+            f32x4 wp, wn;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                wp[v] = k_out[v] ? op[v] : 0.0f;
+                wn[v] = k_out[v] ? on[v] : 0.0f;
+            }
+            const auto rp = rsrc(a.pos_out, p), rn = rsrc(a.neg_out, p);
+            const int so = row_off(r);
+            if (k0 + 4 <= pitch) {
+                __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, wp), rp, vo_store, so, 0);
+                __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, wn), rn, vo_store, so, 0);
+            } else {
+                const f32x2 hp = {wp[0], wp[1]}, hn = {wn[0], wn[1]};
+                __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, hp), rp, vo_store, so, 0);
+                __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, hn), rn, vo_store, so, 0);
+            }
+        }
     };
     // T2 of output plane p from T1 in registers and LDS buffer b; stored
     auto t2_plane = [&](int p, int b) {
@@ -247,24 +288,9 @@ __global__ void __launch_bounds__(1024) sweep2_kernel(Sweep2Args a) {
             double rec[4];
             update(u_cur_p, u_cur_n, jmp, jmn, jpp, jpn, u_prev_p, u_prev_n, u_next_p, u_next_n, op, on, rec);
             if (inside(p)) score(op, on, rec, s2r, s2p, s2n);
-            if (k_out[0] || k_out[1] || k_out[2] || k_out[3]) {
-                f32x4 wp, wn;
-#pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    wp[v] = k_out[v] ? op[v] : 0.0f;
-                    wn[v] = k_out[v] ? on[v] : 0.0f;
-                }
-                const auto rp = rsrc(a.pos_out, p), rn = rsrc(a.neg_out, p);
-                const int so = row_off(r);
-                if (k0 + 4 <= pitch) {
-                    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, wp), rp, vo_store, so, 0);
-                    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, wn), rn, vo_store, so, 0);
-                } else {
-                    const f32x2 hp = {wp[0], wp[1]}, hn = {wn[0], wn[1]};
-                    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, hp), rp, vo_store, so, 0);
-                    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, hn), rn, vo_store, so, 0);
-                }
-            }
+            store_row(p, op, on);
+        } else if (ring_store) {
+            store_row(p, u_cur_p, u_cur_n);  // the carried ring row at step t+2
         }
     };
 
@@ -288,7 +314,10 @@ __global__ void __launch_bounds__(1024) sweep2_kernel(Sweep2Args a) {
     u_prev_p = u_cur_p;
     u_prev_n = u_cur_n;
     t1_plane(i0, tp[1], tn[1], tp[2], tn[2], tp[3], tn[3], i0 & 1, true, u_cur_p, u_cur_n);
-    if (i0 - 1 == 0 && t1_row) t1_ring_plane(0, u_prev_p, u_prev_n);
+    if (i0 - 1 == 0 && (t1_row || (!reflect && ring_row))) {
+        t1_ring_plane(0, tp[1], tn[1], u_prev_p, u_prev_n);
+        if (!reflect && (out_row || ring_store)) store_row(0, u_prev_p, u_prev_n);  // the carried ring plane
+    }
 
     // the march: output plane p, step-t planes p .. p+3 in slots o+2 .. o+5 (mod 4)
     auto step = [&](int p, auto Oc) {
@@ -302,8 +331,10 @@ __global__ void __launch_bounds__(1024) sweep2_kernel(Sweep2Args a) {
         __syncthreads();
         if (p + 1 <= no_xy - 2)
             t1_plane(p + 1, tp[S0], tn[S0], tp[S1], tn[S1], tp[S2], tn[S2], (p + 1) & 1, p + 1 < i1, u_next_p, u_next_n);
-        else if (t1_row)
-            t1_ring_plane(p + 1, u_next_p, u_next_n);
+        else if (t1_row || (!reflect && ring_row)) {
+            t1_ring_plane(p + 1, tp[S1], tn[S1], u_next_p, u_next_n);
+            if (!reflect && (out_row || ring_store)) store_row(p + 1, u_next_p, u_next_n);  // the carried ring plane
+        }
         t2_plane(p, p & 1);
         u_prev_p = u_cur_p;
         u_prev_n = u_cur_n;

@@ -164,8 +164,8 @@ class HipSimulation:
                    "pic_set_tiling")
 
     def set_steps_per_pass(self, steps: int = 0, planes_per_chunk: int = 0) -> None:
-        """1 or 2 time steps per sweep launch (2: FP32 fields, reflecting
-        wall; fields bit-identical to single steps); 0 for the library's
+        """1 or 2 time steps per sweep launch (2: FP32 fields, either
+        lateral wall; fields bit-identical to single steps); 0 for the library's
         choice. ``planes_per_chunk`` sets the two-step pass's planes per block."""
         _lib.check(self.L.pic_set_steps_per_pass(self.handle, int(steps), int(planes_per_chunk)),
                    "pic_set_steps_per_pass")
